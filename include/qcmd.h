@@ -1,0 +1,111 @@
+/*
+ * qcmd.h -- C-ABI of libqcmd.so, the MI355X (gfx950) McMurchie-Davidson
+ * integral and Fock-build library.
+ *
+ * Conventions: every entry point returns 0 on success and a nonzero status
+ * otherwise, with qc_last_error() describing the failure (the calling
+ * thread's last error).  All array arguments are device pointers
+ * (caller-owned, e.g. torch tensors' data_ptr()) unless documented as host;
+ * `stream` is a hipStream_t (NULL = default stream).  Nothing here
+ * allocates device memory except qc_init (per-device Boys tables).
+ *
+ * The reference (jax_qc, Python) has no FFI: its boundary is a set of Python
+ * functions.  Each entry point below names the reference function it
+ * replaces; the Python layer (jax_qc package in jax-examples_amd/) keeps the
+ * reference's names and signatures on top of these.
+ */
+#ifndef QCMD_H
+#define QCMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-device setup: builds the FP64/FP32 Boys tables on `device`.  Must be
+ * called once per device before any other entry point on it. */
+int qc_init(int device);
+const char* qc_last_error(void);
+
+/* F_0..F_nmax at each of n points: F[i*(nmax+1)+k] = F_k(t[i]).
+ * Replaces jax_qc/integrals/boys.py:58-109 (boys_f0, boys_fn). */
+int qc_boys(int nmax, const double* t, double* F, int64_t n, void* stream);
+int qc_boys_f32(int nmax, const float* t, float* F, int64_t n, void* stream);
+
+/* Primitive-pair table record (12 reals): p, P(3), P-A(3), P-B(3),
+ * K = c_a c_b exp(-ab/p |A-B|^2), a (exponent on the first centre). */
+#define QC_PP_STRIDE 12
+
+/* Arguments of one ERI class launch (see qc_eri). */
+typedef struct {
+  const double* pairs64;  /* [npp][12] primitive-pair table, FP64 */
+  const float* pairs32;   /* same in FP32 (FP32 launches only) */
+  const int* sp_prim_off; /* per shell pair: first primitive pair */
+  const int* sp_nprim;    /* per shell pair: primitive-pair count */
+  const int* sp_ao;       /* per shell pair: (first AO of shell 1, of shell 2) */
+  const int* bra_list;    /* shell-pair ids of the bra pair class */
+  const int* ket_list;    /* shell-pair ids of the ket pair class */
+  const int64_t* offsets; /* [nbra+1] prefix sum of surviving kets per bra */
+  const int* kstart;      /* [nbra] first ket position per bra */
+  int nbra;
+  int64_t nquartets;
+  int mode;               /* 0 dense n^4 tensor, 1 packed supermatrix, 2 Schwarz */
+  int64_t nbf;
+  double* out;            /* dense tensor or tile-major supermatrix (mode 0/1) */
+  double* schwarz;        /* per shell pair bound (mode 2) */
+  const double* boys64;   /* filled in by the library */
+  const float* boys32;
+} qc_eri_args;
+
+/* One angular-momentum class of shell-quartet ERIs.  cab, ccd are pair-class
+ * ids la(la+1)/2+lb with la>=lb, cab>=ccd, l<=2; fp32 selects FP32 primitive
+ * sums.  Replaces jax_qc/integrals/eri.py:69-138 (compute_eri_tensor),
+ * obara_saika.py:227-332/394-426 (primitive/contracted_eri_block) and, in
+ * mode 2, screening.py:32-58 (compute_shell_pair_bounds). */
+int qc_eri(int cab, int ccd, int fp32, const qc_eri_args* args, void* stream);
+
+/* One-electron S, T, V (which = 0, 1, 2) for all shell pairs of pair class
+ * cab, written (both triangles) into the dense n x n matrix M.
+ * Replaces overlap.py:67-85, kinetic.py:42-59, nuclear.py:62-81. */
+int qc_one_electron(int which, int cab, const double* pairs64, const int* sp_prim_off,
+                    const int* sp_nprim, const int* sp_ao, const int* pair_list, int npairs,
+                    int natom, const double* atom_xyz, const double* atom_z, int64_t nbf,
+                    double* M, void* stream);
+
+/* Supermatrix post-processing.  kind 0: P = J - 1/4[(ik|jl)+(il|jk)] (RHF
+ * Raffenetti supermatrix); kind 1: K = 1/2[(ik|jl)+(il|jk)].  Input and
+ * output are tile-major packed supermatrices over nbf functions. */
+int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* stream);
+
+/* Workspace (bytes) for qc_symv over nbf functions and nrhs vectors. */
+int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs);
+
+/* y[r] = M x[r] for r < nrhs with M the symmetric tile-major supermatrix;
+ * x, y are [nrhs][npair_pad] packed-pair vectors. */
+int qc_symv(const double* M, int64_t nbf, int nrhs, const double* x, double* y, void* work,
+            void* stream);
+
+/* The two phases of qc_symv, callable separately (e.g. to time the
+ * HBM-streaming tile pass alone): partial sums into `work`, then y. */
+int qc_symv_tiles(const double* M, int64_t nbf, int nrhs, const double* x, void* work,
+                  void* stream);
+int qc_symv_reduce(int64_t nbf, int nrhs, double* y, void* work, void* stream);
+
+/* x[r][pair(k,l)] = D_r[k][l] * (k == l ? 1 : 2) for r < nrhs. */
+int qc_pack_density(const double* D, int64_t nbf, int nrhs, double* x, void* stream);
+
+/* F_r = H + scale * unpack(y_r) (symmetric), r < nrhs (H may be NULL). */
+int qc_unpack_fock(const double* H, const double* y, double scale, int64_t nbf, int nrhs,
+                   double* F, void* stream);
+
+/* J_mn = sum_ls D_ls (mn|ls), K_mn = sum_ls D_ls (ms|ln) from a dense
+ * tensor.  Replaces jax_qc/scf/fock.py:31-38 (_coulomb, _exchange). */
+int qc_dense_jk(const double* eri, const double* D, int64_t nbf, double* J, double* K,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QCMD_H */

@@ -1,0 +1,129 @@
+"""Build libqcmd.so for gfx950 from the HIP sources in this directory.
+
+Each ERI class (and precision) is its own translation unit, generated into
+``_build/gen``, so the 42 heavily unrolled kernels compile in parallel.
+Usage: ``python build.py [--jobs N] [--reuse]``; ``--reuse`` keeps objects
+whose sources (including headers) are unchanged.
+"""
+
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import hashlib
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+REPO = os.path.dirname(PKG)
+BUILD = os.path.join(PKG, "_build")
+GEN = os.path.join(BUILD, "gen")
+OBJ = os.path.join(BUILD, "obj")
+LIB_DIR = os.path.join(PKG, "jax_qc", "_lib")
+LIB = os.path.join(LIB_DIR, "libqcmd.so")
+
+ERI_CLASSES = [
+    (0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 0), (1, 1, 1, 1),
+    (2, 0, 0, 0), (2, 0, 1, 0), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 0, 0), (2, 1, 1, 0),
+    (2, 1, 1, 1), (2, 1, 2, 0), (2, 1, 2, 1), (2, 2, 0, 0), (2, 2, 1, 0), (2, 2, 1, 1),
+    (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2),
+]
+STATIC_SOURCES = ["capi.hip", "oneint.hip", "jk.hip", "grad.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
+    "-I" + HERE, "-I" + os.path.join(REPO, "include"),
+    # no packed FP32 instructions (v_pk_*_f32) anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _generate():
+    os.makedirs(GEN, exist_ok=True)
+    out = []
+    for c in ERI_CLASSES:
+        tag = "".join(map(str, c))
+        for prec, real in (("f64", "double"), ("f32", "float")):
+            path = os.path.join(GEN, f"eri_{tag}_{prec}.hip")
+            text = (
+                '#include "eri_impl.h"\n'
+                "namespace qcmd {\n"
+                f"hipError_t eri_launch_{tag}_{prec}(const EriArgs& a, hipStream_t s) {{\n"
+                f"  return eri_launch_class<{c[0]}, {c[1]}, {c[2]}, {c[3]}, {real}>(a, s);\n"
+                "}\n}  // namespace qcmd\n"
+            )
+            if not os.path.exists(path) or open(path).read() != text:
+                with open(path, "w") as fh:
+                    fh.write(text)
+            # rough cost for scheduling: total angular momentum, fp64 first
+            out.append((sum(c) * 2 + (prec == "f64"), path))
+    return out
+
+
+def _headers_digest():
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(HERE)):
+        if name.endswith(".h"):
+            h.update(open(os.path.join(HERE, name), "rb").read())
+    h.update(open(os.path.join(REPO, "include", "qcmd.h"), "rb").read())
+    h.update(" ".join(FLAGS).encode())
+    return h.hexdigest()
+
+
+def _compile(src, reuse, hdr):
+    os.makedirs(OBJ, exist_ok=True)
+    obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
+    digest = hashlib.sha256(open(src, "rb").read() + hdr.encode()).hexdigest()
+    stamp = obj + ".sha"
+    if reuse and os.path.exists(obj) and os.path.exists(stamp) and open(stamp).read() == digest:
+        return obj, 0.0, True
+    t = time.time()
+    r = subprocess.run([HIPCC, *FLAGS, "-c", src, "-o", obj], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-4000:]}")
+    with open(stamp, "w") as fh:
+        fh.write(digest)
+    return obj, time.time() - t, False
+
+
+def build(jobs: int | None = None, reuse: bool = False, verbose: bool = True) -> str:
+    t0 = time.time()
+    gen = sorted(_generate(), reverse=True)
+    srcs = [p for _, p in gen] + [os.path.join(HERE, s) for s in STATIC_SOURCES
+                                  if os.path.exists(os.path.join(HERE, s))]
+    hdr = _headers_digest()
+    jobs = jobs or max(1, os.cpu_count() or 1)
+    objs = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(_compile, s, reuse, hdr): s for s in srcs}
+        for f in cf.as_completed(futs):
+            obj, dt, cached = f.result()
+            objs.append(obj)
+            if verbose and not cached:
+                print(f"  [{time.time() - t0:6.1f}s] {os.path.basename(obj)} {dt:.1f}s", flush=True)
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *sorted(objs)],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB} in {time.time() - t0:.1f}s", flush=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--reuse", action="store_true")
+    a = ap.parse_args()
+    try:
+        build(a.jobs, a.reuse)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

@@ -1,0 +1,335 @@
+// jk.hip -- Fock build from integrals held in HBM.
+//
+// The reference contracts a dense (n,n,n,n) tensor with two einsums per Fock
+// build (jax_qc/scf/fock.py:31-79).  Here the unique integrals live in a
+// packed supermatrix over basis-function pairs p = i(i+1)/2 + j (i >= j),
+// tile-major in 64 x 64 tiles (lower triangle of tiles, diagonal tiles
+// full).  With D~_kl = D_kl (2 - delta_kl):
+//   J_ij = sum_{k>=l} (ij|kl) D~_kl
+//   K_ij = sum_{k>=l} 1/2[(ik|jl) + (il|jk)] D~_kl
+//   RHF: F = H + sum_{k>=l} P[(ij),(kl)] D~_kl,  P = (ij|kl) - 1/4[(ik|jl)+(il|jk)]
+// (Raffenetti's supermatrix), so each Fock build is one symmetric
+// matrix-vector product that streams every unique integral once from HBM.
+#include "qcmd_device.h"
+#include "qcmd_internal.h"
+#include "qcmd_types.h"
+
+namespace qcmd {
+
+constexpr int SYMV_SEG = 16;  // tiles per work item
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+__host__ __device__ inline int64_t sm_tiles_per_dim(int64_t nbf) {
+  int64_t npair = nbf * (nbf + 1) / 2;
+  return (npair + SM_B - 1) / SM_B;
+}
+
+__device__ inline int64_t tri_root(int64_t t) {  // largest R with R(R+1)/2 <= t
+  int64_t R = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while (R * (R + 1) / 2 > t) --R;
+  while ((R + 1) * (R + 2) / 2 <= t) ++R;
+  return R;
+}
+
+__device__ inline int64_t sm_pos2(int64_t p, int64_t q) {  // any order
+  if (p < q) {
+    int64_t t = p;
+    p = q;
+    q = t;
+  }
+  int64_t R = p >> SM_LOGB, C = q >> SM_LOGB;
+  return ((R * (R + 1) / 2 + C) << (2 * SM_LOGB)) + ((p & (SM_B - 1)) << SM_LOGB) + (q & (SM_B - 1));
+}
+
+__device__ inline int64_t pidx(int64_t i, int64_t j) {
+  return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
+__global__ void sm_assemble_kernel(int kind, const double* __restrict__ J, double* __restrict__ out,
+                                   int64_t nbf, int64_t nelem) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  const int64_t npair = nbf * (nbf + 1) / 2;
+  const int64_t t = e >> (2 * SM_LOGB);
+  const int64_t r = (e >> SM_LOGB) & (SM_B - 1), c = e & (SM_B - 1);
+  const int64_t R = tri_root(t), C = t - R * (R + 1) / 2;
+  int64_t p = R * SM_B + r, q = C * SM_B + c;
+  if (p >= npair || q >= npair) {
+    out[e] = 0.0;
+    return;
+  }
+  if (p < q) {
+    int64_t tmp = p;
+    p = q;
+    q = tmp;
+  }
+  const int64_t i = tri_root(p), j = p - i * (i + 1) / 2;
+  const int64_t k = tri_root(q), l = q - k * (k + 1) / 2;
+  const double jv = J[sm_pos2(p, q)];
+  const double v1 = J[sm_pos2(pidx(i, k), pidx(j, l))];
+  const double v2 = J[sm_pos2(pidx(i, l), pidx(j, k))];
+  const double kv = 0.5 * (v1 + v2);
+  out[e] = kind == 0 ? jv - 0.5 * kv : kv;
+}
+
+// Work item m of the SYMV: tile row R, segment s covering tiles
+// [s*SEG, min(s*SEG+SEG, R+1)).  Row R has R/SEG + 1 segments, so
+// item_off(R) = R + SEG*m(m-1)/2 + m*(R - m*SEG) with m = R/SEG.
+__host__ __device__ inline int64_t item_off(int64_t R) {
+  const int64_t m = R / SYMV_SEG;
+  return R + SYMV_SEG * m * (m - 1) / 2 + m * (R - m * SYMV_SEG);
+}
+
+// One wave per work item.  Lane = (ro, cp): columns 2cp, 2cp+1 of each tile;
+// rows 2k+ro, k < 32.  Column partials of a tile are complete within the
+// wave (one xor-32 exchange); row partials accumulate over the segment and
+// are reduced once at the end by recursive halving.
+template <int NRHS>
+__global__ __launch_bounds__(64) void symv_tiles_kernel(const double* __restrict__ M, int64_t T,
+                                                        const double* __restrict__ x, int64_t npad,
+                                                        double* __restrict__ rowbuf,
+                                                        double* __restrict__ colbuf) {
+  const int64_t item = blockIdx.x;
+  int64_t lo = 0, hi = T - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (item_off(mid) <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int64_t R = lo, s = item - item_off(R);
+  const int64_t C0 = s * SYMV_SEG, C1 = min(C0 + SYMV_SEG, R + 1);
+  const int lane = threadIdx.x, cp = lane & 31, ro = lane >> 5;
+
+  double xr[NRHS][32];
+#pragma unroll
+  for (int h = 0; h < NRHS; ++h)
+#pragma unroll
+    for (int k = 0; k < 32; ++k) xr[h][k] = x[h * npad + R * SM_B + 2 * k + ro];
+  double rowacc[NRHS][32];
+#pragma unroll
+  for (int h = 0; h < NRHS; ++h)
+#pragma unroll
+    for (int k = 0; k < 32; ++k) rowacc[h][k] = 0.0;
+
+  for (int64_t C = C0; C < C1; ++C) {
+    const int64_t tile = R * (R + 1) / 2 + C;
+    const dvec2* tp = reinterpret_cast<const dvec2*>(M + (tile << (2 * SM_LOGB)));
+    double xc0[NRHS], xc1[NRHS], col0[NRHS], col1[NRHS];
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h) {
+      xc0[h] = x[h * npad + C * SM_B + 2 * cp];
+      xc1[h] = x[h * npad + C * SM_B + 2 * cp + 1];
+      col0[h] = col1[h] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      const dvec2 m = __builtin_nontemporal_load(tp + (2 * k + ro) * (SM_B / 2) + cp);
+#pragma unroll
+      for (int h = 0; h < NRHS; ++h) {
+        rowacc[h][k] += m.x * xc0[h] + m.y * xc1[h];
+        col0[h] += m.x * xr[h][k];
+        col1[h] += m.y * xr[h][k];
+      }
+    }
+    if (C != R) {
+#pragma unroll
+      for (int h = 0; h < NRHS; ++h) {
+        col0[h] += __shfl_xor(col0[h], 32);
+        col1[h] += __shfl_xor(col1[h], 32);
+        if (ro == 0) {
+          dvec2 v;
+          v.x = col0[h];
+          v.y = col1[h];
+          reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B)[cp] = v;
+        }
+      }
+    }
+  }
+  // recursive halving over the 32 lanes sharing ro: lane cp ends with row 2cp+ro
+#pragma unroll
+  for (int h = 0; h < NRHS; ++h) {
+#pragma unroll
+    for (int step = 0; step < 5; ++step) {
+      const int m = 16 >> step, half = 16 >> step;
+      const bool up = (cp & m) != 0;
+#pragma unroll
+      for (int i = 0; i < half; ++i) {
+        const double send = up ? rowacc[h][i] : rowacc[h][i + half];
+        const double keep = up ? rowacc[h][i + half] : rowacc[h][i];
+        rowacc[h][i] = keep + __shfl_xor(send, m);
+      }
+    }
+    rowbuf[(item * NRHS + h) * SM_B + 2 * cp + ro] = rowacc[h][0];
+  }
+}
+
+// y[h][R*64 + r] = sum of the row partials of row R's items plus the column
+// partials of tiles (R', R), R' > R.
+__global__ void symv_reduce_kernel(int nrhs, int64_t T, int64_t npad,
+                                   const double* __restrict__ rowbuf,
+                                   const double* __restrict__ colbuf, double* __restrict__ y) {
+  const int64_t R = blockIdx.x;
+  const int h = blockIdx.y;
+  const int r = threadIdx.x;
+  double acc = 0.0;
+  const int64_t i0 = item_off(R), i1 = item_off(R + 1);
+  for (int64_t it = i0; it < i1; ++it) acc += rowbuf[(it * nrhs + h) * SM_B + r];
+  for (int64_t Rp = R + 1; Rp < T; ++Rp) {
+    const int64_t tile = Rp * (Rp + 1) / 2 + R;
+    acc += colbuf[(tile * nrhs + h) * SM_B + r];
+  }
+  y[h * npad + R * SM_B + r] = acc;
+}
+
+__global__ void pack_density_kernel(const double* __restrict__ D, int64_t nbf, int nrhs,
+                                    double* __restrict__ x, int64_t npad) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int h = blockIdx.y;
+  if (p >= npad) return;
+  const int64_t npair = nbf * (nbf + 1) / 2;
+  double v = 0.0;
+  if (p < npair) {
+    const int64_t k = tri_root(p), l = p - k * (k + 1) / 2;
+    v = D[(h * nbf + k) * nbf + l] * (k == l ? 1.0 : 2.0);
+  }
+  x[h * npad + p] = v;
+}
+
+__global__ void unpack_fock_kernel(const double* __restrict__ H, const double* __restrict__ y,
+                                   double scale, int64_t nbf, int64_t npad,
+                                   double* __restrict__ F) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int h = blockIdx.y;
+  if (e >= nbf * nbf) return;
+  const int64_t i = e / nbf, j = e % nbf;
+  double v = scale * y[h * npad + pidx(i, j)];
+  if (H) v += H[e];
+  F[h * nbf * nbf + e] = v;
+}
+
+// Dense J, K: one block per (m, n), reduction over (l, s).
+__global__ __launch_bounds__(256) void dense_jk_kernel(const double* __restrict__ eri,
+                                                       const double* __restrict__ D, int64_t n,
+                                                       double* __restrict__ J,
+                                                       double* __restrict__ K) {
+  const int64_t m = blockIdx.x / n, nn = blockIdx.x % n;
+  double sj = 0.0, sk = 0.0;
+  for (int64_t ls = threadIdx.x; ls < n * n; ls += blockDim.x) {
+    const int64_t l = ls / n, s = ls % n;
+    const double d = D[ls];
+    sj += d * eri[((m * n + nn) * n + l) * n + s];
+    sk += d * eri[((m * n + s) * n + l) * n + nn];
+  }
+  __shared__ double bj[256], bk[256];
+  bj[threadIdx.x] = sj;
+  bk[threadIdx.x] = sk;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      bj[threadIdx.x] += bj[threadIdx.x + w];
+      bk[threadIdx.x] += bk[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    J[m * n + nn] = bj[0];
+    K[m * n + nn] = bk[0];
+  }
+}
+
+}  // namespace qcmd
+
+using namespace qcmd;
+
+extern "C" {
+
+int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* stream) {
+  if (kind != 0 && kind != 1) return set_error("qc_sm_assemble: kind %d", kind);
+  if (nbf <= 0) return set_error("qc_sm_assemble: nbf %lld", (long long)nbf);
+  if (Jsm == out) return set_error("qc_sm_assemble: in-place assembly is not supported");
+  const int64_t T = sm_tiles_per_dim(nbf);
+  const int64_t nelem = (T * (T + 1) / 2) << (2 * SM_LOGB);
+  hipLaunchKernelGGL(sm_assemble_kernel, dim3((unsigned)((nelem + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, kind, Jsm, out, nbf, nelem);
+  return check(hipGetLastError(), "qc_sm_assemble launch");
+}
+
+int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs) {
+  const int64_t T = sm_tiles_per_dim(nbf);
+  const int64_t items = item_off(T), tiles = T * (T + 1) / 2;
+  return (items + tiles) * nrhs * SM_B * (int64_t)sizeof(double);
+}
+
+static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const double* x,
+                      double* y, void* work, void* stream) {
+  if (nrhs < 1 || nrhs > 4) return set_error("qc_symv: nrhs %d outside [1, 4]", nrhs);
+  if (nbf <= 0) return set_error("qc_symv: nbf %lld", (long long)nbf);
+  if (!M || !x || !work || (phase != 1 && !y)) return set_error("qc_symv: null pointer");
+  const int64_t T = sm_tiles_per_dim(nbf), npad = T * SM_B;
+  const int64_t items = item_off(T);
+  double* rowbuf = (double*)work;
+  double* colbuf = rowbuf + items * nrhs * SM_B;
+  hipStream_t s = (hipStream_t)stream;
+  if (phase & 1) {
+    switch (nrhs) {
+#define QC_S(N)                                                                                 \
+  case N:                                                                                       \
+    hipLaunchKernelGGL((symv_tiles_kernel<N>), dim3((unsigned)items), dim3(64), 0, s, M, T, x, \
+                       npad, rowbuf, colbuf);                                                   \
+    break;
+      QC_S(1) QC_S(2) QC_S(3) QC_S(4)
+#undef QC_S
+    }
+    if (check(hipGetLastError(), "qc_symv tiles")) return 1;
+  }
+  if (phase & 2) {
+    hipLaunchKernelGGL(symv_reduce_kernel, dim3((unsigned)T, nrhs), dim3(SM_B), 0, s, nrhs, T,
+                       npad, rowbuf, colbuf, y);
+    if (check(hipGetLastError(), "qc_symv reduce")) return 1;
+  }
+  return 0;
+}
+
+int qc_symv(const double* M, int64_t nbf, int nrhs, const double* x, double* y, void* work,
+            void* stream) {
+  return symv_phase(3, M, nbf, nrhs, x, y, work, stream);
+}
+
+int qc_symv_tiles(const double* M, int64_t nbf, int nrhs, const double* x, void* work,
+                  void* stream) {
+  return symv_phase(1, M, nbf, nrhs, x, nullptr, work, stream);
+}
+
+int qc_symv_reduce(int64_t nbf, int nrhs, double* y, void* work, void* stream) {
+  // the reduction reads only the workspace; pass work as a dummy M/x
+  return symv_phase(2, (const double*)work, nbf, nrhs, (const double*)work, y, work, stream);
+}
+
+int qc_pack_density(const double* D, int64_t nbf, int nrhs, double* x, void* stream) {
+  if (nrhs < 1) return set_error("qc_pack_density: nrhs %d", nrhs);
+  const int64_t npad = sm_tiles_per_dim(nbf) * SM_B;
+  hipLaunchKernelGGL(pack_density_kernel, dim3((unsigned)((npad + 255) / 256), nrhs), dim3(256), 0,
+                     (hipStream_t)stream, D, nbf, nrhs, x, npad);
+  return check(hipGetLastError(), "qc_pack_density");
+}
+
+int qc_unpack_fock(const double* H, const double* y, double scale, int64_t nbf, int nrhs,
+                   double* F, void* stream) {
+  if (nrhs < 1) return set_error("qc_unpack_fock: nrhs %d", nrhs);
+  const int64_t npad = sm_tiles_per_dim(nbf) * SM_B;
+  hipLaunchKernelGGL(unpack_fock_kernel, dim3((unsigned)((nbf * nbf + 255) / 256), nrhs),
+                     dim3(256), 0, (hipStream_t)stream, H, y, scale, nbf, npad, F);
+  return check(hipGetLastError(), "qc_unpack_fock");
+}
+
+int qc_dense_jk(const double* eri, const double* D, int64_t nbf, double* J, double* K,
+                void* stream) {
+  if (nbf <= 0 || nbf > 4096) return set_error("qc_dense_jk: nbf %lld", (long long)nbf);
+  hipLaunchKernelGGL(dense_jk_kernel, dim3((unsigned)(nbf * nbf)), dim3(256), 0,
+                     (hipStream_t)stream, eri, D, nbf, J, K);
+  return check(hipGetLastError(), "qc_dense_jk");
+}
+
+}  // extern "C"

@@ -1,0 +1,288 @@
+"""Host planning for the device integral kernels.
+
+* ``PairTable``: shell pairs (i >= j in basis order, each oriented so the
+  first shell has the larger l) grouped by pair class la(la+1)/2 + lb, and the
+  primitive-pair records (p, P, P-A, P-B, K, alpha) the kernels read.  This
+  is the per-pair data the reference recomputes inside every primitive
+  quartet (jax_qc/integrals/obara_saika.py:237-250).
+* ``schwarz``: Q_pair = sqrt(max_ab (ab|ab)) on the device (screening.py:32-58).
+* ``QuartetPlan``: for each class pair (bra class >= ket class) the surviving
+  quartets after Schwarz screening Q_bra Q_ket >= tau, without materialising
+  quartet lists: pairs are sorted by Q (descending) within each class, so the
+  survivors of a bra are a prefix of the ket list; the kernel maps a global
+  index to (bra, ket) through the per-bra prefix sums.  With a precision split
+  the FP64 quartets (bound >= tau32) are the first k64 kets of each bra and
+  the FP32 quartets the rest.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Tuple
+
+import numpy as np
+import torch
+
+from jax_qc import _native as nat
+from jax_qc.basis.build import shell_arrays
+
+MODE_DENSE, MODE_SUPER, MODE_SCHWARZ = 0, 1, 2
+SM_B = 64
+
+
+def pair_class(la: int, lb: int) -> int:
+    return la * (la + 1) // 2 + lb
+
+
+CLASS_L = {0: (0, 0), 1: (1, 0), 2: (1, 1), 3: (2, 0), 4: (2, 1), 5: (2, 2)}
+
+
+class PairTable:
+    """Shell-pair and primitive-pair tables for one basis on one device."""
+
+    def __init__(self, basis, dev=None, fp32: bool = True, prim_cut: float = 0.0):
+        if not basis.spherical:
+            raise NotImplementedError("the device path implements the spherical layout")
+        sa = shell_arrays(basis)
+        if int(sa["l"].max(initial=0)) > 2:
+            raise NotImplementedError("l > 2 shells are not supported (spherical.py:64-67)")
+        self.basis = basis
+        self.nbf = int(basis.n_basis)
+        self.nshell = len(basis.shells)
+        self.dev = dev or nat.device()
+        L, npr, off, ex, co, ce, ao = (sa[k] for k in ("l", "nprim", "prim_off", "exps", "coefs",
+                                                        "centers", "ao_off"))
+        pairs, sp_off, sp_n, sp_ao, sp_cls, sp_sh = [], [], [], [], [], []
+        npp = 0
+        for i in range(self.nshell):
+            for j in range(i + 1):
+                a, b = (i, j) if L[i] >= L[j] else (j, i)
+                A, B = ce[a], ce[b]
+                ea = ex[off[a]:off[a] + npr[a]]
+                eb = ex[off[b]:off[b] + npr[b]]
+                ca = co[off[a]:off[a] + npr[a]]
+                cb = co[off[b]:off[b] + npr[b]]
+                al = np.repeat(ea, len(eb))
+                be = np.tile(eb, len(ea))
+                p = al + be
+                P = (al[:, None] * A + be[:, None] * B) / p[:, None]
+                AB2 = float(np.dot(A - B, A - B))
+                K = np.repeat(ca, len(cb)) * np.tile(cb, len(ca)) * np.exp(-al * be / p * AB2)
+                rec = np.empty((len(p), 12))
+                rec[:, 0] = p
+                rec[:, 1:4] = P
+                rec[:, 4:7] = P - A
+                rec[:, 7:10] = P - B
+                rec[:, 10] = K
+                rec[:, 11] = al
+                if prim_cut > 0.0:
+                    rec = rec[np.abs(K) >= prim_cut]
+                pairs.append(rec)
+                sp_off.append(npp)
+                sp_n.append(len(rec))
+                npp += len(rec)
+                sp_ao.append((ao[a], ao[b]))
+                sp_cls.append(pair_class(int(L[a]), int(L[b])))
+                sp_sh.append((a, b))
+        rec = np.concatenate(pairs) if pairs else np.zeros((0, 12))
+        self.npairs = len(sp_n)
+        self.sp_class = np.array(sp_cls, dtype=np.int32)
+        self.sp_shells = np.array(sp_sh, dtype=np.int32).reshape(-1, 2)
+        self.sp_ao_host = np.array(sp_ao, dtype=np.int32).reshape(-1, 2)
+        self.nprim_pairs = npp
+        d = self.dev
+        self.pairs64 = torch.from_numpy(np.ascontiguousarray(rec)).to(d)
+        self.pairs32 = self.pairs64.float() if fp32 else None
+        self.sp_prim_off = torch.tensor(sp_off, dtype=torch.int32, device=d)
+        self.sp_nprim = torch.tensor(sp_n, dtype=torch.int32, device=d)
+        self.sp_ao = torch.from_numpy(self.sp_ao_host.copy()).to(d)
+        self.class_pairs: Dict[int, np.ndarray] = {
+            c: np.nonzero(self.sp_class == c)[0].astype(np.int32) for c in range(6)}
+        self.Q = None  # per shell pair Schwarz bound (host numpy)
+        self._keep = []
+
+    # -- launch helpers ---------------------------------------------------
+    def base_args(self, mode: int, out=None, schwarz=None) -> nat.QcEriArgs:
+        a = nat.QcEriArgs()
+        a.pairs64 = self.pairs64.data_ptr()
+        a.pairs32 = self.pairs32.data_ptr() if self.pairs32 is not None else 0
+        a.sp_prim_off = self.sp_prim_off.data_ptr()
+        a.sp_nprim = self.sp_nprim.data_ptr()
+        a.sp_ao = self.sp_ao.data_ptr()
+        a.mode = mode
+        a.nbf = self.nbf
+        a.out = out.data_ptr() if out is not None else 0
+        a.schwarz = schwarz.data_ptr() if schwarz is not None else 0
+        return a
+
+
+def schwarz(pt: PairTable) -> np.ndarray:
+    """Per-shell-pair Q = sqrt(max_ab (ab|ab)) (Cartesian diagonal)."""
+    nat.ensure_init(pt.dev)
+    Qd = torch.zeros(pt.npairs, dtype=torch.float64, device=pt.dev)
+    for c in range(6):
+        lst = pt.class_pairs[c]
+        if len(lst) == 0:
+            continue
+        bl = torch.from_numpy(lst).to(pt.dev)
+        offs = torch.arange(len(lst) + 1, dtype=torch.int64, device=pt.dev)
+        ks = torch.arange(len(lst), dtype=torch.int32, device=pt.dev)
+        a = pt.base_args(MODE_SCHWARZ, schwarz=Qd)
+        a.bra_list = a.ket_list = bl.data_ptr()
+        a.offsets = offs.data_ptr()
+        a.kstart = ks.data_ptr()
+        a.nbra = len(lst)
+        a.nquartets = len(lst)
+        nat.call("qc_eri", c, c, 0, ctypes.byref(a), nat.stream_ptr())
+    pt.Q = Qd.cpu().numpy()
+    return pt.Q
+
+
+@dataclass
+class ClassLaunch:
+    cab: int
+    ccd: int
+    fp32: int
+    nbra: int
+    nquartets: int
+    bra: torch.Tensor
+    ket: torch.Tensor
+    offsets: torch.Tensor
+    kstart: torch.Tensor
+
+
+@dataclass
+class QuartetPlan:
+    launches: List[ClassLaunch] = field(default_factory=list)
+    n64: int = 0
+    n32: int = 0
+    n_screened: int = 0
+    tau: float = 0.0
+    tau32: float = 0.0
+    # surviving quartets per class pair and precision, for FLOP accounting
+    counts: Dict[Tuple[int, int, int], int] = field(default_factory=dict)
+
+
+def plan_quartets(pt: PairTable, tau: float = 1e-12, tau32: float | None = None) -> QuartetPlan:
+    """Screened quartet launches for every class pair.  tau32=None: all
+    surviving quartets in FP64; otherwise quartets with bound < tau32 go to
+    the FP32 kernels."""
+    if pt.Q is None:
+        schwarz(pt)
+    Q = pt.Q
+    plan = QuartetPlan(tau=tau, tau32=tau32 or 0.0)
+    sorted_lists, sorted_q = {}, {}
+    for c in range(6):
+        lst = pt.class_pairs[c]
+        order = np.argsort(-Q[lst], kind="stable")
+        sorted_lists[c] = lst[order]
+        sorted_q[c] = Q[lst][order]
+    total = 0
+    for cab in range(6):
+        for ccd in range(cab + 1):
+            bl, kl = sorted_lists[cab], sorted_lists[ccd]
+            if len(bl) == 0 or len(kl) == 0:
+                continue
+            qb, qk = sorted_q[cab], sorted_q[ccd]
+            neg = -qk  # ascending
+            with np.errstate(divide="ignore"):
+                kmax = np.searchsorted(neg, -(tau / np.maximum(qb, 1e-300)), side="right")
+                k64 = (np.searchsorted(neg, -(tau32 / np.maximum(qb, 1e-300)), side="right")
+                       if tau32 else kmax)
+            total += len(bl) * len(kl) if cab != ccd else len(bl) * (len(bl) + 1) // 2
+            if cab == ccd:
+                cap = np.arange(1, len(bl) + 1)
+                kmax = np.minimum(kmax, cap)
+                k64 = np.minimum(k64, cap)
+            k64 = np.minimum(k64, kmax)
+            for fp32, (start, stop) in ((0, (np.zeros_like(k64), k64)), (1, (k64, kmax))):
+                cnt = (stop - start).astype(np.int64)
+                n = int(cnt.sum())
+                if n == 0:
+                    continue
+                offs = np.zeros(len(bl) + 1, dtype=np.int64)
+                offs[1:] = np.cumsum(cnt)
+                d = pt.dev
+                plan.launches.append(ClassLaunch(
+                    cab, ccd, fp32, len(bl), n,
+                    torch.from_numpy(np.ascontiguousarray(bl)).to(d),
+                    torch.from_numpy(np.ascontiguousarray(kl)).to(d),
+                    torch.from_numpy(offs).to(d),
+                    torch.from_numpy(start.astype(np.int32)).to(d)))
+                plan.counts[(cab, ccd, fp32)] = n
+                if fp32:
+                    plan.n32 += n
+                else:
+                    plan.n64 += n
+    plan.n_screened = total - plan.n64 - plan.n32
+    return plan
+
+
+def run_eri(pt: PairTable, plan: QuartetPlan, mode: int, out: torch.Tensor,
+            only=None) -> None:
+    """Launch every class of ``plan`` writing into ``out`` (dense n^4 tensor
+    or tile-major supermatrix)."""
+    nat.ensure_init(pt.dev)
+    s = nat.stream_ptr()
+    for L in plan.launches:
+        if only is not None and (L.cab, L.ccd) not in only:
+            continue
+        a = pt.base_args(mode, out=out)
+        a.bra_list = L.bra.data_ptr()
+        a.ket_list = L.ket.data_ptr()
+        a.offsets = L.offsets.data_ptr()
+        a.kstart = L.kstart.data_ptr()
+        a.nbra = L.nbra
+        a.nquartets = L.nquartets
+        nat.call("qc_eri", L.cab, L.ccd, L.fp32, ctypes.byref(a), s)
+
+
+# --- FLOP model (algorithmic work per primitive quartet) ------------------
+
+def _nh(L):
+    return (L + 1) * (L + 2) * (L + 3) // 6
+
+
+def _nc(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def flops_per_primitive_quartet(la, lb, lc, ld) -> int:
+    """Boys + prefactor + R recursion + ket and bra Hermite contractions:
+    (40+4L) + 30 + 3(L+1)(L+2)(L+3)(L+4)/24 + 2 N_H(Lab) N_H(Lcd) nc_c nc_d
+    + 2 nc_a nc_b N_H(Lab) nc_c nc_d, bra = the pair with the larger L."""
+    if la + lb < lc + ld:
+        la, lb, lc, ld = lc, ld, la, lb
+    L = la + lb + lc + ld
+    Lab, Lcd = la + lb, lc + ld
+    return ((40 + 4 * L) + 30 + 3 * (L + 1) * (L + 2) * (L + 3) * (L + 4) // 24
+            + 2 * _nh(Lab) * _nh(Lcd) * _nc(lc) * _nc(ld)
+            + 2 * _nc(la) * _nc(lb) * _nh(Lab) * _nc(lc) * _nc(ld))
+
+
+def plan_flops(pt: PairTable, plan: QuartetPlan, nprim_quartet: float | None = None) -> Dict[int, float]:
+    """Algorithmic FLOPs of the plan split by precision (fp32 flag), using
+    the mean primitive-pair count of each pair class."""
+    mean_np = {}
+    sp_n = pt.sp_nprim.cpu().numpy()
+    for c in range(6):
+        lst = pt.class_pairs[c]
+        mean_np[c] = float(sp_n[lst].mean()) if len(lst) else 0.0
+    out = {0: 0.0, 1: 0.0}
+    for (cab, ccd, fp32), n in plan.counts.items():
+        la, lb = CLASS_L[cab]
+        lc, ld = CLASS_L[ccd]
+        out[fp32] += n * mean_np[cab] * mean_np[ccd] * flops_per_primitive_quartet(la, lb, lc, ld)
+    return out
+
+
+def sm_tiles_per_dim(nbf: int) -> int:
+    npair = nbf * (nbf + 1) // 2
+    return (npair + SM_B - 1) // SM_B
+
+
+def sm_numel(nbf: int) -> int:
+    T = sm_tiles_per_dim(nbf)
+    return T * (T + 1) // 2 * SM_B * SM_B

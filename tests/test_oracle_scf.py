@@ -1,0 +1,73 @@
+"""Pins the oracle's SCF restatement (and with it the basis data, AO order,
+normalisation and every integral) against the PySCF energies the reference
+hard-codes in examples/07_rhf_scf.py:31-36 and examples/08_rhf_tier2.py:33-41.
+
+The HF, CH4 and NH3 numbers in the same table are not reproduced at the
+geometries the example states (NH3 is off by 3e-2 Eh, HF by 1e-4 Eh, CH4 by
+2e-5 Eh; a proper 106.7 deg NH3 pyramid lands within 6e-5 of its number),
+so they cannot pin anything and are not used.  The 1e-7 tolerance covers
+the digits by which PySCF's stored basis data differ from the
+Basis-Set-Exchange tables used here."""
+
+import pytest
+
+from jax_qc import molecules as M
+from jax_qc.basis import build_basis_set
+from oracle import mmd, scf
+
+PINS = [
+    ("H2", M.h2, "sto-3g", -1.1167143251),
+    ("He", M.he, "sto-3g", -2.8077839575),
+    ("HeH+", M.heh_cation, "sto-3g", -2.8418364993),
+    ("H3+", M.h3_cation, "sto-3g", -1.2375476999),
+    ("H2O", M.h2o_tier2, "sto-3g", -73.2144728363),
+    ("H2O", M.h2o_tier2, "6-31g", -74.4943924164),
+    ("H2O", M.h2o_tier2, "6-31g**", -74.5428909110),
+]
+
+
+@pytest.mark.parametrize("name,mk,basis,ref", PINS, ids=[f"{p[0]}-{p[2]}" for p in PINS])
+def test_rhf_energy_pins(name, mk, basis, ref):
+    mol = mk()
+    bs = build_basis_set(mol, basis)
+    r = scf.run_rhf(mol, mmd.compute_integrals(mol, bs))
+    assert r["converged"]
+    assert r["energy"] == pytest.approx(ref, abs=1e-7)
+
+
+def test_fock_zero_density_is_hcore():
+    mol = M.h2o_tier2()
+    ints = mmd.compute_integrals(mol, build_basis_set(mol, "sto-3g"))
+    import numpy as np
+
+    Z = np.zeros_like(ints["S"])
+    np.testing.assert_allclose(scf.build_fock_rhf(ints["H_core"], Z, ints["ERI"]), ints["H_core"])
+    Fa, Fb = scf.build_fock_uhf(ints["H_core"], Z, Z, ints["ERI"])
+    np.testing.assert_allclose(Fa, ints["H_core"])
+
+
+def test_uhf_half_densities_equal_rhf():
+    import numpy as np
+
+    mol = M.h2o_tier2()
+    ints = mmd.compute_integrals(mol, build_basis_set(mol, "sto-3g"))
+    r = scf.run_rhf(mol, ints)
+    Fa, Fb = scf.build_fock_uhf(ints["H_core"], r["D"] / 2, r["D"] / 2, ints["ERI"])
+    np.testing.assert_allclose(Fa, scf.build_fock_rhf(ints["H_core"], r["D"], ints["ERI"]), atol=1e-12)
+
+
+def test_uhf_closed_shell_equals_rhf():
+    mol = M.h2o_tier2()
+    ints = mmd.compute_integrals(mol, build_basis_set(mol, "sto-3g"))
+    assert scf.run_uhf(mol, ints)["energy"] == pytest.approx(scf.run_rhf(mol, ints)["energy"], abs=1e-8)
+
+
+def test_density_trace_and_energy_weighted_density():
+    import numpy as np
+
+    mol = M.h2o_tier2()
+    ints = mmd.compute_integrals(mol, build_basis_set(mol, "sto-3g"))
+    r = scf.run_rhf(mol, ints)
+    assert float(np.trace(r["D"] @ ints["S"])) == pytest.approx(mol.n_electrons, abs=1e-10)
+    W = scf.energy_weighted_density(r["C"], r["eps"], r["n_occ"])
+    assert float(np.trace(W @ ints["S"])) == pytest.approx(2 * r["eps"][: r["n_occ"]].sum(), abs=1e-10)
