@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", action="store_true",
                     help="skip the SCF and CPU legs (for short profiler runs)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run a plain device copy of this many MiB first, as the byte "
+                         "calibration kernel for HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the quartet "
@@ -123,6 +126,13 @@ def main():
         D = torch.eye(n, dtype=torch.float64, device=dev) * 0.1
     H = ints.H_core.contiguous()
 
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib * 2 ** 17, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     work, x, y = store._buffers(1)
     Fout = torch.empty_like(H)
     s = nat.stream_ptr
