@@ -56,6 +56,7 @@ typedef struct {
   double* schwarz;        /* per shell pair bound (mode 2) */
   const double* boys64;   /* filled in by the library */
   const float* boys32;
+  int variant;            /* 0 built-in choice, 1 one thread per quartet, 2 one wave per quartet */
 } qc_eri_args;
 
 /* One angular-momentum class of shell-quartet ERIs.  cab, ccd are pair-class

@@ -49,11 +49,15 @@ def _generate():
         tag = "".join(map(str, c))
         for prec, real in (("f64", "double"), ("f32", "float")):
             path = os.path.join(GEN, f"eri_{tag}_{prec}.hip")
+            args = f"{c[0]}, {c[1]}, {c[2]}, {c[3]}, {real}"
             text = (
-                '#include "eri_impl.h"\n'
+                '#include "eri_coop.h"\n'
                 "namespace qcmd {\n"
                 f"hipError_t eri_launch_{tag}_{prec}(const EriArgs& a, hipStream_t s) {{\n"
-                f"  return eri_launch_class<{c[0]}, {c[1]}, {c[2]}, {c[3]}, {real}>(a, s);\n"
+                f"  return eri_launch_class<{args}>(a, s);\n"
+                "}\n"
+                f"hipError_t eri_launchc_{tag}_{prec}(const EriArgs& a, hipStream_t s) {{\n"
+                f"  return eri_launch_coop<{args}>(a, s);\n"
                 "}\n}  // namespace qcmd\n"
             )
             if not os.path.exists(path) or open(path).read() != text:

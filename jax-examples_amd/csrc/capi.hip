@@ -110,9 +110,11 @@ static int launch_boys(int nmax, const Real* t, Real* F, int64_t n, hipStream_t 
 }
 
 // ERI class launchers, one per translation unit (generated by build.py).
-#define QC_DECL(A, B, C, D)                                                  \
-  hipError_t eri_launch_##A##B##C##D##_f64(const EriArgs& a, hipStream_t s); \
-  hipError_t eri_launch_##A##B##C##D##_f32(const EriArgs& a, hipStream_t s);
+#define QC_DECL(A, B, C, D)                                                   \
+  hipError_t eri_launch_##A##B##C##D##_f64(const EriArgs& a, hipStream_t s);  \
+  hipError_t eri_launch_##A##B##C##D##_f32(const EriArgs& a, hipStream_t s);  \
+  hipError_t eri_launchc_##A##B##C##D##_f64(const EriArgs& a, hipStream_t s); \
+  hipError_t eri_launchc_##A##B##C##D##_f32(const EriArgs& a, hipStream_t s);
 QC_ERI_CLASSES(QC_DECL)
 #undef QC_DECL
 
@@ -172,11 +174,16 @@ int qc_eri(int cab, int ccd, int fp32, const qc_eri_args* args_in, void* stream)
   if (a.mode == ERI_SCHWARZ && cab != ccd) return set_error("qc_eri: Schwarz mode needs cab == ccd");
   if (a.nquartets < 0 || (a.nquartets > 0 && a.nbra <= 0))
     return set_error("qc_eri: nquartets %lld with nbra %d", (long long)a.nquartets, a.nbra);
+  if (a.variant < 0 || a.variant > 2) return set_error("qc_eri: variant %d", a.variant);
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
-#define QC_CALL(A, B, C, D)                                                   \
-  if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D)) { \
-    e = fp32 ? eri_launch_##A##B##C##D##_f32(a, s) : eri_launch_##A##B##C##D##_f64(a, s); \
+#define QC_CALL(A, B, C, D)                                                                    \
+  if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D)) {                  \
+    const bool coop = a.variant == 2 || (a.variant == 0 && (A) + (B) + (C) + (D) >= 5);        \
+    if (coop)                                                                                  \
+      e = fp32 ? eri_launchc_##A##B##C##D##_f32(a, s) : eri_launchc_##A##B##C##D##_f64(a, s); \
+    else                                                                                       \
+      e = fp32 ? eri_launch_##A##B##C##D##_f32(a, s) : eri_launch_##A##B##C##D##_f64(a, s);   \
   }
   QC_ERI_CLASSES(QC_CALL)
 #undef QC_CALL

@@ -32,6 +32,7 @@ class QcEriArgs(ctypes.Structure):
         ("ket_list", c_void_p), ("offsets", c_void_p), ("kstart", c_void_p),
         ("nbra", c_int), ("nquartets", c_i64), ("mode", c_int), ("nbf", c_i64),
         ("out", c_void_p), ("schwarz", c_void_p), ("boys64", c_void_p), ("boys32", c_void_p),
+        ("variant", c_int),
     ]
 
 

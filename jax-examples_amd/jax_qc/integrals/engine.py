@@ -31,6 +31,18 @@ from jax_qc.basis.build import shell_arrays
 MODE_DENSE, MODE_SUPER, MODE_SCHWARZ = 0, 1, 2
 SM_B = 64
 
+# Kernel variant per (bra class, ket class): 0 = the library's built-in
+# choice, 1 = one thread per quartet, 2 = one wave per quartet.  Filled in
+# by jax_qc.autotune (or by hand); ``FORCE_VARIANT`` overrides every class.
+ERI_VARIANT: Dict[Tuple[int, int], int] = {}
+FORCE_VARIANT: int | None = None
+
+
+def variant_for(cab: int, ccd: int) -> int:
+    if FORCE_VARIANT is not None:
+        return FORCE_VARIANT
+    return ERI_VARIANT.get((cab, ccd), 0)
+
 
 def pair_class(la: int, lb: int) -> int:
     return la * (la + 1) // 2 + lb
@@ -135,6 +147,7 @@ def schwarz(pt: PairTable) -> np.ndarray:
         a.kstart = ks.data_ptr()
         a.nbra = len(lst)
         a.nquartets = len(lst)
+        a.variant = variant_for(c, c)
         nat.call("qc_eri", c, c, 0, ctypes.byref(a), nat.stream_ptr())
     pt.Q = Qd.cpu().numpy()
     return pt.Q
@@ -236,6 +249,7 @@ def run_eri(pt: PairTable, plan: QuartetPlan, mode: int, out: torch.Tensor,
         a.kstart = L.kstart.data_ptr()
         a.nbra = L.nbra
         a.nquartets = L.nquartets
+        a.variant = variant_for(L.cab, L.ccd)
         nat.call("qc_eri", L.cab, L.ccd, L.fp32, ctypes.byref(a), s)
 
 

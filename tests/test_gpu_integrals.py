@@ -83,20 +83,34 @@ def test_one_electron_parity(case):
     np.testing.assert_allclose(compute_nuclear_matrix(bs, mol).cpu().numpy(), ref["V"], atol=1e-11)
 
 
-def test_dense_eri_parity(case):
+@pytest.mark.parametrize("variant", [0, 1, 2], ids=["default", "per-thread", "per-wave"])
+def test_dense_eri_parity(case, variant):
     from jax_qc.integrals import compute_eri_tensor
+    from jax_qc.integrals import engine as eng
 
     mol, bs, ref = case
-    got = compute_eri_tensor(bs).cpu().numpy()
+    eng.FORCE_VARIANT = variant
+    try:
+        got = compute_eri_tensor(bs).cpu().numpy()
+        got32 = compute_eri_tensor(bs, tau32=np.inf).cpu().numpy()
+    finally:
+        eng.FORCE_VARIANT = None
     np.testing.assert_allclose(got, ref["ERI"], atol=1e-12, rtol=0)
+    assert np.abs(got32 - ref["ERI"]).max() <= 5e-6 * max(1.0, np.abs(ref["ERI"]).max())
 
 
-def test_schwarz_parity(case):
+@pytest.mark.parametrize("variant", [1, 2], ids=["per-thread", "per-wave"])
+def test_schwarz_parity(case, variant):
     from jax_qc.integrals import compute_shell_pair_bounds
+    from jax_qc.integrals import engine as eng
 
     mol, bs, ref = case
-    np.testing.assert_allclose(compute_shell_pair_bounds(bs), mmd.shell_pair_bounds(bs),
-                               rtol=1e-12, atol=1e-14)
+    eng.FORCE_VARIANT = variant
+    try:
+        got = compute_shell_pair_bounds(bs)
+    finally:
+        eng.FORCE_VARIANT = None
+    np.testing.assert_allclose(got, mmd.shell_pair_bounds(bs), rtol=1e-12, atol=1e-14)
 
 
 def test_fp32_eri_error_is_bounded(case):
