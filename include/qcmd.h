@@ -93,6 +93,16 @@ int qc_symv_tiles(const double* M, int64_t nbf, int nrhs, const double* x, void*
                   void* stream);
 int qc_symv_reduce(int64_t nbf, int nrhs, double* y, void* work, void* stream);
 
+/* Mixed-precision supermatrix: per-tile max |element| (tmax[T(T+1)/2]),
+ * packing into a pool under a caller-built directory (fmt[t] = 0 skip,
+ * 1 FP32, 2 FP64; off[t] = byte offset in the pool), and the tile pass of the
+ * Fock build over that pool (accumulation in FP64; qc_symv_reduce finishes). */
+int qc_sm_tile_maxabs(const double* M, int64_t nbf, double* tmax, void* stream);
+int qc_sm_pack(const double* M, int64_t nbf, const uint8_t* fmt, const int64_t* off, void* pool,
+               void* stream);
+int qc_symv_mixed_tiles(const void* pool, const uint8_t* fmt, const int64_t* off, int64_t nbf,
+                        int nrhs, const double* x, void* work, void* stream);
+
 /* x[r][pair(k,l)] = D_r[k][l] * (k == l ? 1 : 2) for r < nrhs. */
 int qc_pack_density(const double* D, int64_t nbf, int nrhs, double* x, void* stream);
 

@@ -84,8 +84,73 @@ __host__ __device__ inline int64_t item_off(int64_t R) {
 // rows 2k+ro, k < 32.  Column partials of a tile are complete within the
 // wave (one xor-32 exchange); row partials accumulate over the segment and
 // are reduced once at the end by recursive halving.
-template <int NRHS>
-__global__ __launch_bounds__(64) void symv_tiles_kernel(const double* __restrict__ M, int64_t T,
+// Tile formats of a compacted (mixed-precision) supermatrix.
+enum TileFmt : uint8_t { TILE_ZERO = 0, TILE_F32 = 1, TILE_F64 = 2 };
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ inline dvec2 load2(const T* p);
+template <>
+__device__ inline dvec2 load2<double>(const double* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(p));
+}
+template <>
+__device__ inline dvec2 load2<float>(const float* p) {
+  const fvec2 f = __builtin_nontemporal_load(reinterpret_cast<const fvec2*>(p));
+  dvec2 d;
+  d.x = (double)f.x;
+  d.y = (double)f.y;
+  return d;
+}
+
+// One tile's contribution: row partials into rowacc, column partials
+// (complete after one xor-32 exchange) to colbuf unless the tile is diagonal.
+template <int NRHS, typename T>
+__device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile, bool diag,
+                                          const double (&xr)[NRHS][32], double (&rowacc)[NRHS][32],
+                                          const double* __restrict__ x, int64_t npad, int64_t C,
+                                          int cp, int ro, double* __restrict__ colbuf) {
+  double xc0[NRHS], xc1[NRHS], col0[NRHS], col1[NRHS];
+#pragma unroll
+  for (int h = 0; h < NRHS; ++h) {
+    xc0[h] = x[h * npad + C * SM_B + 2 * cp];
+    xc1[h] = x[h * npad + C * SM_B + 2 * cp + 1];
+    col0[h] = col1[h] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    const dvec2 m = load2<T>(tp + (2 * k + ro) * SM_B + 2 * cp);
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h) {
+      rowacc[h][k] += m.x * xc0[h] + m.y * xc1[h];
+      col0[h] += m.x * xr[h][k];
+      col1[h] += m.y * xr[h][k];
+    }
+  }
+  if (!diag) {
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h) {
+      col0[h] += __shfl_xor(col0[h], 32);
+      col1[h] += __shfl_xor(col1[h], 32);
+      if (ro == 0) {
+        dvec2 v;
+        v.x = col0[h];
+        v.y = col1[h];
+        reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B)[cp] = v;
+      }
+    }
+  }
+}
+
+// One wave per work item.  Lane = (ro, cp): columns 2cp, 2cp+1 of each tile;
+// rows 2k+ro, k < 32.  Column partials of a tile are complete within the
+// wave (one xor-32 exchange); row partials accumulate over the segment and
+// are reduced once at the end by recursive halving.  MIXED: tiles are read
+// through a format/offset directory (zero tiles skipped, FP32 or FP64).
+template <int NRHS, bool MIXED>
+__global__ __launch_bounds__(64) void symv_tiles_kernel(const void* __restrict__ Mv, int64_t T,
+                                                        const uint8_t* __restrict__ fmt,
+                                                        const int64_t* __restrict__ off,
                                                         const double* __restrict__ x, int64_t npad,
                                                         double* __restrict__ rowbuf,
                                                         double* __restrict__ colbuf) {
@@ -115,34 +180,25 @@ __global__ __launch_bounds__(64) void symv_tiles_kernel(const double* __restrict
 
   for (int64_t C = C0; C < C1; ++C) {
     const int64_t tile = R * (R + 1) / 2 + C;
-    const dvec2* tp = reinterpret_cast<const dvec2*>(M + (tile << (2 * SM_LOGB)));
-    double xc0[NRHS], xc1[NRHS], col0[NRHS], col1[NRHS];
+    const bool diag = C == R;
+    if (!MIXED) {
+      const double* tp = reinterpret_cast<const double*>(Mv) + (tile << (2 * SM_LOGB));
+      symv_tile<NRHS, double>(tp, tile, diag, xr, rowacc, x, npad, C, cp, ro, colbuf);
+    } else {
+      const uint8_t f = fmt[tile];
+      const char* base = reinterpret_cast<const char*>(Mv) + off[tile];
+      if (f == TILE_F64) {
+        symv_tile<NRHS, double>(reinterpret_cast<const double*>(base), tile, diag, xr, rowacc, x,
+                                npad, C, cp, ro, colbuf);
+      } else if (f == TILE_F32) {
+        symv_tile<NRHS, float>(reinterpret_cast<const float*>(base), tile, diag, xr, rowacc, x,
+                               npad, C, cp, ro, colbuf);
+      } else if (!diag && ro == 0) {
 #pragma unroll
-    for (int h = 0; h < NRHS; ++h) {
-      xc0[h] = x[h * npad + C * SM_B + 2 * cp];
-      xc1[h] = x[h * npad + C * SM_B + 2 * cp + 1];
-      col0[h] = col1[h] = 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-      const dvec2 m = __builtin_nontemporal_load(tp + (2 * k + ro) * (SM_B / 2) + cp);
-#pragma unroll
-      for (int h = 0; h < NRHS; ++h) {
-        rowacc[h][k] += m.x * xc0[h] + m.y * xc1[h];
-        col0[h] += m.x * xr[h][k];
-        col1[h] += m.y * xr[h][k];
-      }
-    }
-    if (C != R) {
-#pragma unroll
-      for (int h = 0; h < NRHS; ++h) {
-        col0[h] += __shfl_xor(col0[h], 32);
-        col1[h] += __shfl_xor(col1[h], 32);
-        if (ro == 0) {
-          dvec2 v;
-          v.x = col0[h];
-          v.y = col1[h];
-          reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B)[cp] = v;
+        for (int h = 0; h < NRHS; ++h) {
+          dvec2 z;
+          z.x = z.y = 0.0;
+          reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B)[cp] = z;
         }
       }
     }
@@ -162,6 +218,34 @@ __global__ __launch_bounds__(64) void symv_tiles_kernel(const double* __restrict
       }
     }
     rowbuf[(item * NRHS + h) * SM_B + 2 * cp + ro] = rowacc[h][0];
+  }
+}
+
+// Per-tile max |element| (one wave per tile).
+__global__ __launch_bounds__(64) void tile_maxabs_kernel(const double* __restrict__ M,
+                                                         double* __restrict__ tmax) {
+  const int64_t tile = blockIdx.x;
+  const double* tp = M + (tile << (2 * SM_LOGB));
+  double m = 0.0;
+  for (int i = threadIdx.x; i < SM_B * SM_B; i += 64) m = fmax(m, fabs(tp[i]));
+  for (int w = 32; w > 0; w >>= 1) m = fmax(m, __shfl_xor(m, w));
+  if (threadIdx.x == 0) tmax[tile] = m;
+}
+
+// Copy each tile into the compacted pool in its format.
+__global__ __launch_bounds__(256) void tile_pack_kernel(const double* __restrict__ M,
+                                                        const uint8_t* __restrict__ fmt,
+                                                        const int64_t* __restrict__ off,
+                                                        char* __restrict__ pool) {
+  const int64_t tile = blockIdx.x;
+  const uint8_t f = fmt[tile];
+  if (f == TILE_ZERO) return;
+  const double* tp = M + (tile << (2 * SM_LOGB));
+  for (int i = threadIdx.x; i < SM_B * SM_B; i += 256) {
+    if (f == TILE_F64)
+      reinterpret_cast<double*>(pool + off[tile])[i] = tp[i];
+    else
+      reinterpret_cast<float*>(pool + off[tile])[i] = (float)tp[i];
   }
 }
 
@@ -263,7 +347,8 @@ int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs) {
 }
 
 static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const double* x,
-                      double* y, void* work, void* stream) {
+                      double* y, void* work, void* stream, const uint8_t* fmt = nullptr,
+                      const int64_t* off = nullptr) {
   if (nrhs < 1 || nrhs > 4) return set_error("qc_symv: nrhs %d outside [1, 4]", nrhs);
   if (nbf <= 0) return set_error("qc_symv: nbf %lld", (long long)nbf);
   if (!M || !x || !work || (phase != 1 && !y)) return set_error("qc_symv: null pointer");
@@ -274,10 +359,14 @@ static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const d
   hipStream_t s = (hipStream_t)stream;
   if (phase & 1) {
     switch (nrhs) {
-#define QC_S(N)                                                                                 \
-  case N:                                                                                       \
-    hipLaunchKernelGGL((symv_tiles_kernel<N>), dim3((unsigned)items), dim3(64), 0, s, M, T, x, \
-                       npad, rowbuf, colbuf);                                                   \
+#define QC_S(N)                                                                                \
+  case N:                                                                                      \
+    if (fmt)                                                                                   \
+      hipLaunchKernelGGL((symv_tiles_kernel<N, true>), dim3((unsigned)items), dim3(64), 0, s, \
+                         (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf);               \
+    else                                                                                       \
+      hipLaunchKernelGGL((symv_tiles_kernel<N, false>), dim3((unsigned)items), dim3(64), 0, s, \
+                         (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf);               \
     break;
       QC_S(1) QC_S(2) QC_S(3) QC_S(4)
 #undef QC_S
@@ -300,6 +389,27 @@ int qc_symv(const double* M, int64_t nbf, int nrhs, const double* x, double* y, 
 int qc_symv_tiles(const double* M, int64_t nbf, int nrhs, const double* x, void* work,
                   void* stream) {
   return symv_phase(1, M, nbf, nrhs, x, nullptr, work, stream);
+}
+
+int qc_symv_mixed_tiles(const void* pool, const uint8_t* fmt, const int64_t* off, int64_t nbf,
+                        int nrhs, const double* x, void* work, void* stream) {
+  if (!fmt || !off) return set_error("qc_symv_mixed_tiles: null tile directory");
+  return symv_phase(1, (const double*)pool, nbf, nrhs, x, nullptr, work, stream, fmt, off);
+}
+
+int qc_sm_tile_maxabs(const double* M, int64_t nbf, double* tmax, void* stream) {
+  const int64_t T = sm_tiles_per_dim(nbf);
+  hipLaunchKernelGGL(tile_maxabs_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(64), 0,
+                     (hipStream_t)stream, M, tmax);
+  return check(hipGetLastError(), "qc_sm_tile_maxabs");
+}
+
+int qc_sm_pack(const double* M, int64_t nbf, const uint8_t* fmt, const int64_t* off, void* pool,
+               void* stream) {
+  const int64_t T = sm_tiles_per_dim(nbf);
+  hipLaunchKernelGGL(tile_pack_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0,
+                     (hipStream_t)stream, M, fmt, off, (char*)pool);
+  return check(hipGetLastError(), "qc_sm_pack");
 }
 
 int qc_symv_reduce(int64_t nbf, int nrhs, double* y, void* work, void* stream) {

@@ -14,6 +14,7 @@ from jax_qc.core.types import (  # noqa: F401
     Shell,
     make_molecule,
 )
+from jax_qc import autotune  # noqa: F401  (installs the measured per-class kernel choice)
 from jax_qc.integrals import compute_integrals  # noqa: F401
 from jax_qc.profiling import StageTimer  # noqa: F401
 from jax_qc.scf import run_rhf, run_scf, run_uhf  # noqa: F401

@@ -47,6 +47,10 @@ _SIGS = {
     "qc_symv": [c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "qc_symv_tiles": [c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p],
     "qc_symv_reduce": [c_i64, c_int, c_void_p, c_void_p, c_void_p],
+    "qc_sm_tile_maxabs": [c_void_p, c_i64, c_void_p, c_void_p],
+    "qc_sm_pack": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p, c_void_p],
+    "qc_symv_mixed_tiles": [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p,
+                            c_void_p],
     "qc_pack_density": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
     "qc_unpack_fock": [c_void_p, c_void_p, c_double, c_i64, c_int, c_void_p, c_void_p],
     "qc_dense_jk": [c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p],

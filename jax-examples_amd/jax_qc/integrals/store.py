@@ -21,11 +21,33 @@ from __future__ import annotations
 
 import math
 import time
+from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 from jax_qc import _native as nat
 from jax_qc.integrals import engine as eng
+
+
+TILE_ZERO, TILE_F32, TILE_F64 = 0, 1, 2
+
+
+@dataclass
+class CompactSupermatrix:
+    """A supermatrix re-packed tile by tile: tiles whose max |element| is
+    below ``tau_zero`` are dropped, below ``tau_f32`` stored in FP32, the rest
+    in FP64.  The Fock build accumulates in FP64 either way."""
+
+    pool: torch.Tensor      # uint8 byte pool
+    fmt: torch.Tensor       # uint8 per tile
+    off: torch.Tensor       # int64 byte offset per tile
+    n_zero: int
+    n_f32: int
+    n_f64: int
+    bytes: int
+    tau_f32: float
+    tau_zero: float
 
 
 class ERIStore:
@@ -52,6 +74,16 @@ class ERIStore:
         self.P = self.K = None
         self._work, self._x, self._y = {}, {}, {}
 
+    def use_compact(self, tau_f32: float = 1e-4, tau_zero: float = 1e-14):
+        """Replace P (and J/K if present) by their compacted mixed forms."""
+        self.ensure_P()
+        self.P = self.compact(self.P, tau_f32, tau_zero) if isinstance(self.P, torch.Tensor) else self.P
+        if self.K is not None and isinstance(self.K, torch.Tensor):
+            self.K = self.compact(self.K, tau_f32, tau_zero)
+        if self.J is not None and isinstance(self.J, torch.Tensor) and self.K is not None:
+            self.J = self.compact(self.J, tau_f32, tau_zero)
+        return self
+
     def ensure_P(self):
         if self.P is None:
             self.P = torch.empty_like(self.J)
@@ -70,6 +102,28 @@ class ERIStore:
         """Drop the J supermatrix once only P is needed (RHF)."""
         self.ensure_P()
         self.J = None
+
+    def compact(self, M: torch.Tensor, tau_f32: float = 1e-4, tau_zero: float = 1e-14
+                ) -> CompactSupermatrix:
+        """Mixed-precision, zero-skipping copy of supermatrix M."""
+        T = self.T
+        nt = T * (T + 1) // 2
+        tmax = torch.empty(nt, dtype=torch.float64, device=self.dev)
+        nat.call("qc_sm_tile_maxabs", nat.ptr(M), self.nbf, nat.ptr(tmax), nat.stream_ptr())
+        tm = tmax.cpu().numpy()
+        fmt = np.where(tm < tau_zero, TILE_ZERO, np.where(tm < tau_f32, TILE_F32, TILE_F64))
+        fmt = fmt.astype(np.uint8)
+        size = np.where(fmt == TILE_F64, 8 * 4096, np.where(fmt == TILE_F32, 4 * 4096, 0))
+        off = np.zeros(nt, dtype=np.int64)
+        off[1:] = np.cumsum(size)[:-1]
+        total = int(size.sum())
+        pool = torch.empty(max(16, total), dtype=torch.uint8, device=self.dev)
+        fd = torch.from_numpy(fmt).to(self.dev)
+        od = torch.from_numpy(off).to(self.dev)
+        nat.call("qc_sm_pack", nat.ptr(M), self.nbf, nat.ptr(fd), nat.ptr(od), nat.ptr(pool),
+                 nat.stream_ptr())
+        return CompactSupermatrix(pool, fd, od, int((fmt == 0).sum()), int((fmt == 1).sum()),
+                                  int((fmt == 2).sum()), total, tau_f32, tau_zero)
 
     # -- Fock builds --------------------------------------------------------
     def bytes_per_matrix(self) -> int:
@@ -94,7 +148,12 @@ class ERIStore:
         work, x, y = self._buffers(nrhs)
         s = nat.stream_ptr()
         nat.call("qc_pack_density", nat.ptr(D), self.nbf, nrhs, nat.ptr(x), s)
-        nat.call("qc_symv", nat.ptr(M), self.nbf, nrhs, nat.ptr(x), nat.ptr(y), nat.ptr(work), s)
+        if isinstance(M, CompactSupermatrix):
+            nat.call("qc_symv_mixed_tiles", nat.ptr(M.pool), nat.ptr(M.fmt), nat.ptr(M.off),
+                     self.nbf, nrhs, nat.ptr(x), nat.ptr(work), s)
+            nat.call("qc_symv_reduce", self.nbf, nrhs, nat.ptr(y), nat.ptr(work), s)
+        else:
+            nat.call("qc_symv", nat.ptr(M), self.nbf, nrhs, nat.ptr(x), nat.ptr(y), nat.ptr(work), s)
         F = torch.empty_like(D)
         Hp = H.contiguous() if H is not None else None
         nat.call("qc_unpack_fock", nat.ptr(Hp), nat.ptr(y), float(scale), self.nbf, nrhs,

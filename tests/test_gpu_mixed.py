@@ -1,0 +1,88 @@
+"""Mixed precision: FP32-split ERIs and FP32 supermatrix tiles against the
+FP64 path (bars: tile-exact when every tile is FP64; |dE| <= 1 uEh for the
+default thresholds on a water cluster; FP32 storage error bounded by the
+FP32 rounding of the stored values)."""
+
+import numpy as np
+import pytest
+import torch
+
+from jax_qc import molecules as M
+from jax_qc.basis import build_basis_set
+from jax_qc.core.types import CalcConfig
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dimer():
+    mol = M.water_cluster((2, 1, 1), seed=4)
+    return mol, build_basis_set(mol, "even-tempered")
+
+
+def test_compact_all_fp64_is_exact(dimer):
+    from jax_qc.integrals.store import ERIStore
+
+    mol, bs = dimer
+    st = ERIStore(bs, tau=1e-12)
+    P = st.ensure_P()
+    C = st.compact(P, tau_f32=0.0, tau_zero=0.0)
+    assert C.n_f32 == 0 and C.n_zero == 0
+    n = bs.n_basis
+    D = torch.randn(n, n, dtype=torch.float64, device="cuda")
+    D = D + D.T
+    a = st.contract(P, D)
+    b = st.contract(C, D)
+    assert torch.equal(a, b)
+
+
+def test_compact_fp32_error_bounded(dimer):
+    from jax_qc.integrals.store import ERIStore
+
+    mol, bs = dimer
+    st = ERIStore(bs, tau=1e-12)
+    P = st.ensure_P()
+    C = st.compact(P, tau_f32=np.inf, tau_zero=0.0)
+    assert C.n_f64 == 0
+    n = bs.n_basis
+    D = torch.randn(n, n, dtype=torch.float64, device="cuda")
+    D = D + D.T
+    a = st.contract(P, D)
+    b = st.contract(C, D)
+    # each element rounded to FP32 (rel 6e-8); G_ij sums npair of them
+    bound = 6e-8 * st.contract(st.compact(P.abs(), tau_f32=0.0), D.abs()).max()
+    assert (a - b).abs().max() <= bound
+
+
+def test_mixed_scf_energy_within_1uEh(dimer):
+    import jax_qc
+
+    mol, bs = dimer
+    E = {}
+    for mode in ("fp64", "mixed"):
+        ints = jax_qc.compute_integrals(mol, bs, eri="store", tau=1e-12,
+                                        tau32=1e-4 if mode == "mixed" else None)
+        if mode == "mixed":
+            ints.ERI.use_compact(tau_f32=1e-4)
+            assert ints.ERI.plan.n32 > 0
+        r = jax_qc.run_scf(mol, ints, CalcConfig(method="rhf"))
+        assert r.converged
+        E[mode] = r.energy
+    assert abs(E["mixed"] - E["fp64"]) <= 1e-6
+
+
+def test_autotune_picks_a_valid_variant(dimer):
+    from jax_qc import autotune
+    from jax_qc.integrals import engine as eng
+
+    mol, bs = dimer
+    pt = eng.PairTable(bs)
+    plan = eng.plan_quartets(pt, 1e-12)
+    saved = dict(eng.ERI_VARIANT)
+    try:
+        t = autotune.autotune_eri(pt, plan, repeats=1)
+        assert t and all(set(v) == {1, 2} for v in t.values())
+        assert all(eng.ERI_VARIANT[(a, b)] in (1, 2) for (a, b, _) in t)
+    finally:
+        eng.ERI_VARIANT.clear()
+        eng.ERI_VARIANT.update(saved)
