@@ -79,6 +79,10 @@ int qc_one_electron(int which, int cab, const double* pairs64, const int* sp_pri
  * output are tile-major packed supermatrices over nbf functions. */
 int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* stream);
 
+/* Launch shape of the Fock-build tile pass: tiles per work item (one wave
+ * per item) and whether tile loads are non-temporal.  Process-wide. */
+int qc_symv_config(int seg, int nontemporal);
+
 /* Workspace (bytes) for qc_symv over nbf functions and nrhs vectors. */
 int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs);
 

@@ -16,7 +16,9 @@
 
 namespace qcmd {
 
-constexpr int SYMV_SEG = 16;  // tiles per work item
+// Tiles per SYMV work item and the load flavour; set by qc_symv_config.
+static int g_symv_seg = 16;
+static int g_symv_nt = 1;
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 __host__ __device__ inline int64_t sm_tiles_per_dim(int64_t nbf) {
@@ -73,53 +75,56 @@ __global__ void sm_assemble_kernel(int kind, const double* __restrict__ J, doubl
 }
 
 // Work item m of the SYMV: tile row R, segment s covering tiles
-// [s*SEG, min(s*SEG+SEG, R+1)).  Row R has R/SEG + 1 segments, so
-// item_off(R) = R + SEG*m(m-1)/2 + m*(R - m*SEG) with m = R/SEG.
-__host__ __device__ inline int64_t item_off(int64_t R) {
-  const int64_t m = R / SYMV_SEG;
-  return R + SYMV_SEG * m * (m - 1) / 2 + m * (R - m * SYMV_SEG);
+// [s*seg, min(s*seg+seg, R+1)).  Row R has R/seg + 1 segments, so
+// item_off(R) = R + seg*m(m-1)/2 + m*(R - m*seg) with m = R/seg.
+__host__ __device__ inline int64_t item_off(int64_t R, int seg) {
+  const int64_t m = R / seg;
+  return R + seg * m * (m - 1) / 2 + m * (R - m * seg);
 }
 
-// One wave per work item.  Lane = (ro, cp): columns 2cp, 2cp+1 of each tile;
-// rows 2k+ro, k < 32.  Column partials of a tile are complete within the
-// wave (one xor-32 exchange); row partials accumulate over the segment and
-// are reduced once at the end by recursive halving.
 // Tile formats of a compacted (mixed-precision) supermatrix.
 enum TileFmt : uint8_t { TILE_ZERO = 0, TILE_F32 = 1, TILE_F64 = 2 };
 typedef float fvec2 __attribute__((ext_vector_type(2)));
 
-template <typename T>
-__device__ inline dvec2 load2(const T* p);
-template <>
-__device__ inline dvec2 load2<double>(const double* p) {
-  return __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(p));
-}
-template <>
-__device__ inline dvec2 load2<float>(const float* p) {
-  const fvec2 f = __builtin_nontemporal_load(reinterpret_cast<const fvec2*>(p));
-  dvec2 d;
-  d.x = (double)f.x;
-  d.y = (double)f.y;
-  return d;
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+template <bool NT, typename V>
+__device__ __forceinline__ V ld(const V* p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
 }
 
-// One tile's contribution: row partials into rowacc, column partials
-// (complete after one xor-32 exchange) to colbuf unless the tile is diagonal.
-template <int NRHS, typename T>
-__device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile, bool diag,
-                                          const double (&xr)[NRHS][32], double (&rowacc)[NRHS][32],
-                                          const double* __restrict__ x, int64_t npad, int64_t C,
-                                          int cp, int ro, double* __restrict__ colbuf) {
-  double xc0[NRHS], xc1[NRHS], col0[NRHS], col1[NRHS];
+// Column partials of one off-diagonal tile are complete after one xor-32
+// exchange; lane (cp, ro == 0) stores columns 2cp, 2cp+1.
+template <int NRHS>
+__device__ __forceinline__ void store_cols(double (&col0)[NRHS], double (&col1)[NRHS], int64_t tile,
+                                           int cp, int ro, double* __restrict__ colbuf) {
 #pragma unroll
   for (int h = 0; h < NRHS; ++h) {
-    xc0[h] = x[h * npad + C * SM_B + 2 * cp];
-    xc1[h] = x[h * npad + C * SM_B + 2 * cp + 1];
-    col0[h] = col1[h] = 0.0;
+    col0[h] += __shfl_xor(col0[h], 32);
+    col1[h] += __shfl_xor(col1[h], 32);
+    if (ro == 0) {
+      dvec2 v;
+      v.x = col0[h];
+      v.y = col1[h];
+      reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B)[cp] = v;
+    }
   }
+}
+
+// FP64 tile (row-major): lane reads rows 2k+ro, columns 2cp..2cp+1 (16 B).
+template <int NRHS, bool NT>
+__device__ __forceinline__ void symv_tile64(const double* __restrict__ tp, int64_t tile, bool diag,
+                                            const double (&xr)[NRHS][32],
+                                            double (&rowacc)[NRHS][32],
+                                            const double (&xc0)[NRHS], const double (&xc1)[NRHS],
+                                            int cp, int ro, double* __restrict__ colbuf) {
+  double col0[NRHS], col1[NRHS];
+#pragma unroll
+  for (int h = 0; h < NRHS; ++h) col0[h] = col1[h] = 0.0;
 #pragma unroll
   for (int k = 0; k < 32; ++k) {
-    const dvec2 m = load2<T>(tp + (2 * k + ro) * SM_B + 2 * cp);
+    const dvec2 m = ld<NT>(reinterpret_cast<const dvec2*>(tp + (2 * k + ro) * SM_B + 2 * cp));
 #pragma unroll
     for (int h = 0; h < NRHS; ++h) {
       rowacc[h][k] += m.x * xc0[h] + m.y * xc1[h];
@@ -127,19 +132,34 @@ __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile
       col1[h] += m.y * xr[h][k];
     }
   }
-  if (!diag) {
+  if (!diag) store_cols<NRHS>(col0, col1, tile, cp, ro, colbuf);
+}
+
+// FP32 tile in the lane-permuted layout written by tile_pack_kernel: float4
+// ((kk*2 + ro)*32 + cp) holds rows 4kk+ro and 4kk+2+ro, columns 2cp..2cp+1,
+// so every lane reads 16 B per load as in the FP64 case.
+template <int NRHS, bool NT>
+__device__ __forceinline__ void symv_tile32(const float* __restrict__ tp, int64_t tile, bool diag,
+                                            const double (&xr)[NRHS][32],
+                                            double (&rowacc)[NRHS][32],
+                                            const double (&xc0)[NRHS], const double (&xc1)[NRHS],
+                                            int cp, int ro, double* __restrict__ colbuf) {
+  double col0[NRHS], col1[NRHS];
+#pragma unroll
+  for (int h = 0; h < NRHS; ++h) col0[h] = col1[h] = 0.0;
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk) {
+    const fvec4 f = ld<NT>(reinterpret_cast<const fvec4*>(tp) + (kk * 2 + ro) * 32 + cp);
+    const double m0x = f.x, m0y = f.y, m1x = f.z, m1y = f.w;
 #pragma unroll
     for (int h = 0; h < NRHS; ++h) {
-      col0[h] += __shfl_xor(col0[h], 32);
-      col1[h] += __shfl_xor(col1[h], 32);
-      if (ro == 0) {
-        dvec2 v;
-        v.x = col0[h];
-        v.y = col1[h];
-        reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B)[cp] = v;
-      }
+      rowacc[h][2 * kk] += m0x * xc0[h] + m0y * xc1[h];
+      rowacc[h][2 * kk + 1] += m1x * xc0[h] + m1y * xc1[h];
+      col0[h] += m0x * xr[h][2 * kk] + m1x * xr[h][2 * kk + 1];
+      col1[h] += m0y * xr[h][2 * kk] + m1y * xr[h][2 * kk + 1];
     }
   }
+  if (!diag) store_cols<NRHS>(col0, col1, tile, cp, ro, colbuf);
 }
 
 // One wave per work item.  Lane = (ro, cp): columns 2cp, 2cp+1 of each tile;
@@ -147,24 +167,24 @@ __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile
 // wave (one xor-32 exchange); row partials accumulate over the segment and
 // are reduced once at the end by recursive halving.  MIXED: tiles are read
 // through a format/offset directory (zero tiles skipped, FP32 or FP64).
-template <int NRHS, bool MIXED>
+template <int NRHS, bool MIXED, bool NT>
 __global__ __launch_bounds__(64) void symv_tiles_kernel(const void* __restrict__ Mv, int64_t T,
                                                         const uint8_t* __restrict__ fmt,
                                                         const int64_t* __restrict__ off,
                                                         const double* __restrict__ x, int64_t npad,
                                                         double* __restrict__ rowbuf,
-                                                        double* __restrict__ colbuf) {
+                                                        double* __restrict__ colbuf, int seg) {
   const int64_t item = blockIdx.x;
   int64_t lo = 0, hi = T - 1;
   while (lo < hi) {
     int64_t mid = (lo + hi + 1) >> 1;
-    if (item_off(mid) <= item)
+    if (item_off(mid, seg) <= item)
       lo = mid;
     else
       hi = mid - 1;
   }
-  const int64_t R = lo, s = item - item_off(R);
-  const int64_t C0 = s * SYMV_SEG, C1 = min(C0 + SYMV_SEG, R + 1);
+  const int64_t R = lo, s = item - item_off(R, seg);
+  const int64_t C0 = s * seg, C1 = min(C0 + seg, R + 1);
   const int lane = threadIdx.x, cp = lane & 31, ro = lane >> 5;
 
   double xr[NRHS][32];
@@ -181,18 +201,24 @@ __global__ __launch_bounds__(64) void symv_tiles_kernel(const void* __restrict__
   for (int64_t C = C0; C < C1; ++C) {
     const int64_t tile = R * (R + 1) / 2 + C;
     const bool diag = C == R;
+    double xc0[NRHS], xc1[NRHS];
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h) {
+      xc0[h] = x[h * npad + C * SM_B + 2 * cp];
+      xc1[h] = x[h * npad + C * SM_B + 2 * cp + 1];
+    }
     if (!MIXED) {
       const double* tp = reinterpret_cast<const double*>(Mv) + (tile << (2 * SM_LOGB));
-      symv_tile<NRHS, double>(tp, tile, diag, xr, rowacc, x, npad, C, cp, ro, colbuf);
+      symv_tile64<NRHS, NT>(tp, tile, diag, xr, rowacc, xc0, xc1, cp, ro, colbuf);
     } else {
       const uint8_t f = fmt[tile];
       const char* base = reinterpret_cast<const char*>(Mv) + off[tile];
       if (f == TILE_F64) {
-        symv_tile<NRHS, double>(reinterpret_cast<const double*>(base), tile, diag, xr, rowacc, x,
-                                npad, C, cp, ro, colbuf);
+        symv_tile64<NRHS, NT>(reinterpret_cast<const double*>(base), tile, diag, xr, rowacc, xc0,
+                              xc1, cp, ro, colbuf);
       } else if (f == TILE_F32) {
-        symv_tile<NRHS, float>(reinterpret_cast<const float*>(base), tile, diag, xr, rowacc, x,
-                               npad, C, cp, ro, colbuf);
+        symv_tile32<NRHS, NT>(reinterpret_cast<const float*>(base), tile, diag, xr, rowacc, xc0,
+                              xc1, cp, ro, colbuf);
       } else if (!diag && ro == 0) {
 #pragma unroll
         for (int h = 0; h < NRHS; ++h) {
@@ -242,29 +268,38 @@ __global__ __launch_bounds__(256) void tile_pack_kernel(const double* __restrict
   if (f == TILE_ZERO) return;
   const double* tp = M + (tile << (2 * SM_LOGB));
   for (int i = threadIdx.x; i < SM_B * SM_B; i += 256) {
-    if (f == TILE_F64)
+    if (f == TILE_F64) {
       reinterpret_cast<double*>(pool + off[tile])[i] = tp[i];
-    else
-      reinterpret_cast<float*>(pool + off[tile])[i] = (float)tp[i];
+    } else {
+      // row r = 2k + ro, k = 2kk + j; column c = 2cp + jc
+      const int r = i >> SM_LOGB, c = i & (SM_B - 1);
+      const int ro = r & 1, k = r >> 1, kk = k >> 1, j = k & 1, cp = c >> 1, jc = c & 1;
+      reinterpret_cast<float*>(pool + off[tile])[((kk * 2 + ro) * 32 + cp) * 4 + j * 2 + jc] =
+          (float)tp[i];
+    }
   }
 }
 
 // y[h][R*64 + r] = sum of the row partials of row R's items plus the column
-// partials of tiles (R', R), R' > R.
-__global__ void symv_reduce_kernel(int nrhs, int64_t T, int64_t npad,
-                                   const double* __restrict__ rowbuf,
-                                   const double* __restrict__ colbuf, double* __restrict__ y) {
+// partials of tiles (R', R), R' > R; four waves split the terms.
+__global__ __launch_bounds__(256) void symv_reduce_kernel(int nrhs, int64_t T, int64_t npad,
+                                                          const double* __restrict__ rowbuf,
+                                                          const double* __restrict__ colbuf,
+                                                          double* __restrict__ y, int seg) {
   const int64_t R = blockIdx.x;
   const int h = blockIdx.y;
-  const int r = threadIdx.x;
+  const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
   double acc = 0.0;
-  const int64_t i0 = item_off(R), i1 = item_off(R + 1);
-  for (int64_t it = i0; it < i1; ++it) acc += rowbuf[(it * nrhs + h) * SM_B + r];
-  for (int64_t Rp = R + 1; Rp < T; ++Rp) {
+  const int64_t i0 = item_off(R, seg), i1 = item_off(R + 1, seg);
+  for (int64_t it = i0 + w; it < i1; it += 4) acc += rowbuf[(it * nrhs + h) * SM_B + r];
+  for (int64_t Rp = R + 1 + w; Rp < T; Rp += 4) {
     const int64_t tile = Rp * (Rp + 1) / 2 + R;
     acc += colbuf[(tile * nrhs + h) * SM_B + r];
   }
-  y[h * npad + R * SM_B + r] = acc;
+  __shared__ double part[4][64];
+  part[w][r] = acc;
+  __syncthreads();
+  if (w == 0) y[h * npad + R * SM_B + r] = part[0][r] + part[1][r] + part[2][r] + part[3][r];
 }
 
 __global__ void pack_density_kernel(const double* __restrict__ D, int64_t nbf, int nrhs,
@@ -340,9 +375,17 @@ int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* 
   return check(hipGetLastError(), "qc_sm_assemble launch");
 }
 
+int qc_symv_config(int seg, int nontemporal) {
+  if (seg < 1 || seg > 4096) return set_error("qc_symv_config: seg %d", seg);
+  g_symv_seg = seg;
+  g_symv_nt = nontemporal ? 1 : 0;
+  return 0;
+}
+
 int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs) {
+  // sized for the smallest supported segment so any later qc_symv_config fits
   const int64_t T = sm_tiles_per_dim(nbf);
-  const int64_t items = item_off(T), tiles = T * (T + 1) / 2;
+  const int64_t items = item_off(T, 1), tiles = T * (T + 1) / 2;
   return (items + tiles) * nrhs * SM_B * (int64_t)sizeof(double);
 }
 
@@ -353,29 +396,39 @@ static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const d
   if (nbf <= 0) return set_error("qc_symv: nbf %lld", (long long)nbf);
   if (!M || !x || !work || (phase != 1 && !y)) return set_error("qc_symv: null pointer");
   const int64_t T = sm_tiles_per_dim(nbf), npad = T * SM_B;
-  const int64_t items = item_off(T);
+  const int seg = g_symv_seg;
+  const int64_t items = item_off(T, seg);
   double* rowbuf = (double*)work;
   double* colbuf = rowbuf + items * nrhs * SM_B;
   hipStream_t s = (hipStream_t)stream;
   if (phase & 1) {
     switch (nrhs) {
-#define QC_S(N)                                                                                \
-  case N:                                                                                      \
-    if (fmt)                                                                                   \
-      hipLaunchKernelGGL((symv_tiles_kernel<N, true>), dim3((unsigned)items), dim3(64), 0, s, \
-                         (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf);               \
-    else                                                                                       \
-      hipLaunchKernelGGL((symv_tiles_kernel<N, false>), dim3((unsigned)items), dim3(64), 0, s, \
-                         (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf);               \
+#define QC_L(N, MX, NT)                                                                     \
+  hipLaunchKernelGGL((symv_tiles_kernel<N, MX, NT>), dim3((unsigned)items), dim3(64), 0, s, \
+                     (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf, seg)
+#define QC_S(N)                     \
+  case N:                           \
+    if (fmt) {                      \
+      if (g_symv_nt)                \
+        QC_L(N, true, true);        \
+      else                          \
+        QC_L(N, true, false);       \
+    } else {                        \
+      if (g_symv_nt)                \
+        QC_L(N, false, true);       \
+      else                          \
+        QC_L(N, false, false);      \
+    }                               \
     break;
       QC_S(1) QC_S(2) QC_S(3) QC_S(4)
 #undef QC_S
+#undef QC_L
     }
     if (check(hipGetLastError(), "qc_symv tiles")) return 1;
   }
   if (phase & 2) {
-    hipLaunchKernelGGL(symv_reduce_kernel, dim3((unsigned)T, nrhs), dim3(SM_B), 0, s, nrhs, T,
-                       npad, rowbuf, colbuf, y);
+    hipLaunchKernelGGL(symv_reduce_kernel, dim3((unsigned)T, nrhs), dim3(256), 0, s, nrhs, T,
+                       npad, rowbuf, colbuf, y, seg);
     if (check(hipGetLastError(), "qc_symv reduce")) return 1;
   }
   return 0;

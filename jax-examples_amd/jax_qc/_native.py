@@ -51,6 +51,7 @@ _SIGS = {
     "qc_sm_pack": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p, c_void_p],
     "qc_symv_mixed_tiles": [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p,
                             c_void_p],
+    "qc_symv_config": [c_int, c_int],
     "qc_pack_density": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
     "qc_unpack_fock": [c_void_p, c_void_p, c_double, c_i64, c_int, c_void_p, c_void_p],
     "qc_dense_jk": [c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
