@@ -147,7 +147,7 @@ def main():
     ap.add_argument("--mode", choices=["auto", "fp64", "mixed"], default="auto",
                     help="auto: time both paths, report the mixed rate when its |dE| vs the "
                          "FP64 path is <= 1 uEh, else the FP64 rate")
-    ap.add_argument("--tile-f32", type=float, default=1e-4,
+    ap.add_argument("--tile-f32", type=float, default=1e-2,
                     help="supermatrix tiles with max |element| below this are stored in FP32")
     ap.add_argument("--grid", default="2,2,2")
     ap.add_argument("--no-cpu", action="store_true")

@@ -17,7 +17,7 @@
 namespace qcmd {
 
 // Tiles per SYMV work item and the load flavour; set by qc_symv_config.
-static int g_symv_seg = 16;
+static int g_symv_seg = 4;
 static int g_symv_nt = 1;
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 
@@ -94,81 +94,93 @@ __device__ __forceinline__ V ld(const V* p) {
   return *p;
 }
 
-// Column partials of one off-diagonal tile are complete after one xor-32
-// exchange; lane (cp, ro == 0) stores columns 2cp, 2cp+1.
-template <int NRHS>
-__device__ __forceinline__ void store_cols(double (&col0)[NRHS], double (&col1)[NRHS], int64_t tile,
-                                           int cp, int ro, double* __restrict__ colbuf) {
+// Lane layout of the tile pass: lane = (ro, cg), ro = lane >> 4 in 0..3,
+// cg = lane & 15; the lane covers columns 4cg..4cg+3 of every tile and rows
+// 4k + ro, k < 16.  Both tile formats are row-major, so a wave reads 1 KB
+// (FP32) or 2 KB (FP64) of contiguous tile per row quadruple.
+constexpr int SYMV_ROWS = 16;  // rows per lane
+#ifndef SYMV_MIN_WAVES
+#define SYMV_MIN_WAVES 1  // waves per SIMD the register budget must allow
+#endif
+
+// Column j < 4 of lane cg: FP32 tiles 4cg + j (16 B per row); FP64 tiles
+// 2cg + j for j < 2 and 32 + 2cg + (j-2) otherwise, so each 16-B load of a
+// wave covers 256 contiguous bytes of four rows.
+template <typename T>
+__device__ __forceinline__ int lane_col(int cg, int j) {
+  if (sizeof(T) == 4) return 4 * cg + j;
+  return j < 2 ? 2 * cg + j : 32 + 2 * cg + (j - 2);
+}
+
+// Column partials of one off-diagonal tile: sum over the lane's 16 rows,
+// then over the four row offsets (xor 16, xor 32); lanes with ro == 0 store.
+template <int NRHS, typename T>
+__device__ __forceinline__ void store_cols(double (&col)[NRHS][4], int64_t tile, int cg, int ro,
+                                           double* __restrict__ colbuf) {
 #pragma unroll
   for (int h = 0; h < NRHS; ++h) {
-    col0[h] += __shfl_xor(col0[h], 32);
-    col1[h] += __shfl_xor(col1[h], 32);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      col[h][j] += __shfl_xor(col[h][j], 16);
+      col[h][j] += __shfl_xor(col[h][j], 32);
+    }
     if (ro == 0) {
-      dvec2 v;
-      v.x = col0[h];
-      v.y = col1[h];
-      reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B)[cp] = v;
+      double* dst = colbuf + (tile * NRHS + h) * SM_B;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dst[lane_col<T>(cg, j)] = col[h][j];
     }
   }
 }
 
-// FP64 tile (row-major): lane reads rows 2k+ro, columns 2cp..2cp+1 (16 B).
-template <int NRHS, bool NT>
-__device__ __forceinline__ void symv_tile64(const double* __restrict__ tp, int64_t tile, bool diag,
-                                            const double (&xr)[NRHS][32],
-                                            double (&rowacc)[NRHS][32],
-                                            const double (&xc0)[NRHS], const double (&xc1)[NRHS],
-                                            int cp, int ro, double* __restrict__ colbuf) {
-  double col0[NRHS], col1[NRHS];
+template <int NRHS, bool NT, typename T>
+__device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile, bool diag,
+                                          const double (&xr)[NRHS][SYMV_ROWS],
+                                          double (&rowacc)[NRHS][SYMV_ROWS],
+                                          const double* __restrict__ xcb, int64_t npad, int cg,
+                                          int ro, double* __restrict__ colbuf) {
+  double xc[NRHS][4], col[NRHS][4];
 #pragma unroll
-  for (int h = 0; h < NRHS; ++h) col0[h] = col1[h] = 0.0;
+  for (int h = 0; h < NRHS; ++h)
 #pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    const dvec2 m = ld<NT>(reinterpret_cast<const dvec2*>(tp + (2 * k + ro) * SM_B + 2 * cp));
+    for (int j = 0; j < 4; ++j) {
+      xc[h][j] = xcb[h * npad + lane_col<T>(cg, j)];
+      col[h][j] = 0.0;
+    }
+#pragma unroll
+  for (int k = 0; k < SYMV_ROWS; ++k) {
+    const T* row = tp + (4 * k + ro) * SM_B;
+    double m[4];
+    if (sizeof(T) == 8) {
+      const dvec2 a = ld<NT>(reinterpret_cast<const dvec2*>(row) + cg);
+      const dvec2 b = ld<NT>(reinterpret_cast<const dvec2*>(row) + 16 + cg);
+      m[0] = a.x;
+      m[1] = a.y;
+      m[2] = b.x;
+      m[3] = b.y;
+    } else {
+      const fvec4 f = ld<NT>(reinterpret_cast<const fvec4*>(row) + cg);
+      m[0] = f.x;
+      m[1] = f.y;
+      m[2] = f.z;
+      m[3] = f.w;
+    }
 #pragma unroll
     for (int h = 0; h < NRHS; ++h) {
-      rowacc[h][k] += m.x * xc0[h] + m.y * xc1[h];
-      col0[h] += m.x * xr[h][k];
-      col1[h] += m.y * xr[h][k];
+      rowacc[h][k] += m[0] * xc[h][0] + m[1] * xc[h][1] + m[2] * xc[h][2] + m[3] * xc[h][3];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) col[h][j] += m[j] * xr[h][k];
     }
   }
-  if (!diag) store_cols<NRHS>(col0, col1, tile, cp, ro, colbuf);
+  if (!diag) store_cols<NRHS, T>(col, tile, cg, ro, colbuf);
 }
 
-// FP32 tile in the lane-permuted layout written by tile_pack_kernel: float4
-// ((kk*2 + ro)*32 + cp) holds rows 4kk+ro and 4kk+2+ro, columns 2cp..2cp+1,
-// so every lane reads 16 B per load as in the FP64 case.
-template <int NRHS, bool NT>
-__device__ __forceinline__ void symv_tile32(const float* __restrict__ tp, int64_t tile, bool diag,
-                                            const double (&xr)[NRHS][32],
-                                            double (&rowacc)[NRHS][32],
-                                            const double (&xc0)[NRHS], const double (&xc1)[NRHS],
-                                            int cp, int ro, double* __restrict__ colbuf) {
-  double col0[NRHS], col1[NRHS];
-#pragma unroll
-  for (int h = 0; h < NRHS; ++h) col0[h] = col1[h] = 0.0;
-#pragma unroll
-  for (int kk = 0; kk < 16; ++kk) {
-    const fvec4 f = ld<NT>(reinterpret_cast<const fvec4*>(tp) + (kk * 2 + ro) * 32 + cp);
-    const double m0x = f.x, m0y = f.y, m1x = f.z, m1y = f.w;
-#pragma unroll
-    for (int h = 0; h < NRHS; ++h) {
-      rowacc[h][2 * kk] += m0x * xc0[h] + m0y * xc1[h];
-      rowacc[h][2 * kk + 1] += m1x * xc0[h] + m1y * xc1[h];
-      col0[h] += m0x * xr[h][2 * kk] + m1x * xr[h][2 * kk + 1];
-      col1[h] += m0y * xr[h][2 * kk] + m1y * xr[h][2 * kk + 1];
-    }
-  }
-  if (!diag) store_cols<NRHS>(col0, col1, tile, cp, ro, colbuf);
-}
-
-// One wave per work item.  Lane = (ro, cp): columns 2cp, 2cp+1 of each tile;
-// rows 2k+ro, k < 32.  Column partials of a tile are complete within the
-// wave (one xor-32 exchange); row partials accumulate over the segment and
-// are reduced once at the end by recursive halving.  MIXED: tiles are read
-// through a format/offset directory (zero tiles skipped, FP32 or FP64).
+// One wave per work item (tile row R, a segment of its tiles).  Row
+// partials accumulate in registers over the segment and are reduced once at
+// the end by recursive halving over the 16 lanes sharing ro.  MIXED: tiles
+// are read through a format/offset directory (zero tiles skipped, FP32 or
+// FP64, FP64 accumulation either way).
 template <int NRHS, bool MIXED, bool NT>
-__global__ __launch_bounds__(64) void symv_tiles_kernel(const void* __restrict__ Mv, int64_t T,
+__global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_tiles_kernel(const void* __restrict__ Mv, int64_t T,
                                                         const uint8_t* __restrict__ fmt,
                                                         const int64_t* __restrict__ off,
                                                         const double* __restrict__ x, int64_t npad,
@@ -185,57 +197,52 @@ __global__ __launch_bounds__(64) void symv_tiles_kernel(const void* __restrict__
   }
   const int64_t R = lo, s = item - item_off(R, seg);
   const int64_t C0 = s * seg, C1 = min(C0 + seg, R + 1);
-  const int lane = threadIdx.x, cp = lane & 31, ro = lane >> 5;
+  const int lane = threadIdx.x, cg = lane & 15, ro = lane >> 4;
 
-  double xr[NRHS][32];
+  double xr[NRHS][SYMV_ROWS], rowacc[NRHS][SYMV_ROWS];
 #pragma unroll
   for (int h = 0; h < NRHS; ++h)
 #pragma unroll
-    for (int k = 0; k < 32; ++k) xr[h][k] = x[h * npad + R * SM_B + 2 * k + ro];
-  double rowacc[NRHS][32];
-#pragma unroll
-  for (int h = 0; h < NRHS; ++h)
-#pragma unroll
-    for (int k = 0; k < 32; ++k) rowacc[h][k] = 0.0;
+    for (int k = 0; k < SYMV_ROWS; ++k) {
+      xr[h][k] = x[h * npad + R * SM_B + 4 * k + ro];
+      rowacc[h][k] = 0.0;
+    }
 
   for (int64_t C = C0; C < C1; ++C) {
     const int64_t tile = R * (R + 1) / 2 + C;
     const bool diag = C == R;
-    double xc0[NRHS], xc1[NRHS];
-#pragma unroll
-    for (int h = 0; h < NRHS; ++h) {
-      xc0[h] = x[h * npad + C * SM_B + 2 * cp];
-      xc1[h] = x[h * npad + C * SM_B + 2 * cp + 1];
-    }
+    const double* xcb = x + C * SM_B;
     if (!MIXED) {
       const double* tp = reinterpret_cast<const double*>(Mv) + (tile << (2 * SM_LOGB));
-      symv_tile64<NRHS, NT>(tp, tile, diag, xr, rowacc, xc0, xc1, cp, ro, colbuf);
+      symv_tile<NRHS, NT, double>(tp, tile, diag, xr, rowacc, xcb, npad, cg, ro, colbuf);
     } else {
       const uint8_t f = fmt[tile];
       const char* base = reinterpret_cast<const char*>(Mv) + off[tile];
       if (f == TILE_F64) {
-        symv_tile64<NRHS, NT>(reinterpret_cast<const double*>(base), tile, diag, xr, rowacc, xc0,
-                              xc1, cp, ro, colbuf);
+        symv_tile<NRHS, NT, double>(reinterpret_cast<const double*>(base), tile, diag, xr, rowacc,
+                                    xcb, npad, cg, ro, colbuf);
       } else if (f == TILE_F32) {
-        symv_tile32<NRHS, NT>(reinterpret_cast<const float*>(base), tile, diag, xr, rowacc, xc0,
-                              xc1, cp, ro, colbuf);
+        symv_tile<NRHS, NT, float>(reinterpret_cast<const float*>(base), tile, diag, xr, rowacc,
+                                   xcb, npad, cg, ro, colbuf);
       } else if (!diag && ro == 0) {
 #pragma unroll
         for (int h = 0; h < NRHS; ++h) {
           dvec2 z;
           z.x = z.y = 0.0;
-          reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B)[cp] = z;
+          dvec2* dst = reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B + 4 * cg);
+          dst[0] = z;
+          dst[1] = z;
         }
       }
     }
   }
-  // recursive halving over the 32 lanes sharing ro: lane cp ends with row 2cp+ro
+  // recursive halving over the 16 lanes sharing ro: lane cg ends with row 4cg+ro
 #pragma unroll
   for (int h = 0; h < NRHS; ++h) {
 #pragma unroll
-    for (int step = 0; step < 5; ++step) {
-      const int m = 16 >> step, half = 16 >> step;
-      const bool up = (cp & m) != 0;
+    for (int step = 0; step < 4; ++step) {
+      const int m = 8 >> step, half = 8 >> step;
+      const bool up = (cg & m) != 0;
 #pragma unroll
       for (int i = 0; i < half; ++i) {
         const double send = up ? rowacc[h][i] : rowacc[h][i + half];
@@ -243,7 +250,7 @@ __global__ __launch_bounds__(64) void symv_tiles_kernel(const void* __restrict__
         rowacc[h][i] = keep + __shfl_xor(send, m);
       }
     }
-    rowbuf[(item * NRHS + h) * SM_B + 2 * cp + ro] = rowacc[h][0];
+    rowbuf[(item * NRHS + h) * SM_B + 4 * cg + ro] = rowacc[h][0];
   }
 }
 
@@ -268,15 +275,10 @@ __global__ __launch_bounds__(256) void tile_pack_kernel(const double* __restrict
   if (f == TILE_ZERO) return;
   const double* tp = M + (tile << (2 * SM_LOGB));
   for (int i = threadIdx.x; i < SM_B * SM_B; i += 256) {
-    if (f == TILE_F64) {
+    if (f == TILE_F64)
       reinterpret_cast<double*>(pool + off[tile])[i] = tp[i];
-    } else {
-      // row r = 2k + ro, k = 2kk + j; column c = 2cp + jc
-      const int r = i >> SM_LOGB, c = i & (SM_B - 1);
-      const int ro = r & 1, k = r >> 1, kk = k >> 1, j = k & 1, cp = c >> 1, jc = c & 1;
-      reinterpret_cast<float*>(pool + off[tile])[((kk * 2 + ro) * 32 + cp) * 4 + j * 2 + jc] =
-          (float)tp[i];
-    }
+    else
+      reinterpret_cast<float*>(pool + off[tile])[i] = (float)tp[i];
   }
 }
 
