@@ -119,6 +119,47 @@ int qc_unpack_fock(const double* H, const double* y, double scale, int64_t nbf, 
 int qc_dense_jk(const double* eri, const double* D, int64_t nbf, double* J, double* K,
                 void* stream);
 
+/* ---- analytic nuclear gradient (contracted on the fly) ---------------- */
+
+/* Arguments of one two-electron gradient class launch.  Densities are in
+ * the Cartesian AO basis (ncart x ncart).  G_abcd = Dj_ab Dj_cd -
+ * cx sum_s (Dk_s,ac Dk_s,bd + Dk_s,ad Dk_s,bc); RHF: Dj = Dk0 = D, nk = 1,
+ * cx = 1/4; UHF: Dj = Da + Db, Dk = {Da, Db}, nk = 2, cx = 1/2. */
+typedef struct {
+  const double* pairs64;
+  const int* sp_prim_off;
+  const int* sp_nprim;
+  const int* sp_aoc;   /* per shell pair: Cartesian AO offsets (shell 1, shell 2) */
+  const int* sp_atom;  /* per shell pair: atom of shell 1, atom of shell 2 */
+  const int* sp_diag;  /* per shell pair: 1 when both shells are one shell */
+  const int* bra_list;
+  const int* ket_list;
+  const int64_t* offsets;
+  const int* kstart;
+  int nbra;
+  int64_t nquartets;
+  int64_t ncart;
+  const double* dj;
+  const double* dk0;
+  const double* dk1;
+  int nk;
+  double cx;
+  double* grad;        /* natom x 3, accumulated (atomic adds) */
+} qc_grad2e_args;
+
+/* 1/2 sum_quartets f sum_abcd G_abcd d(ab|cd)/dR for one class pair,
+ * added into grad.  Replaces the dERI term of jax_qc/grad/rhf_grad.py:
+ * 186-209 (which rebuilds all ERIs at R +- h e_k). */
+int qc_grad2e(int cab, int ccd, const qc_grad2e_args* args, void* stream);
+
+/* sum_ab (2 - delta_ij) [D_ab d(T+V)_ab/dR - W_ab dS_ab/dR] over the shell
+ * pairs of class cab (including the nuclear-centre derivative of V), added
+ * into grad.  Replaces the dH and dS terms of rhf_grad.py:186-209. */
+int qc_grad1e(int cab, const double* pairs64, const int* sp_prim_off, const int* sp_nprim,
+              const int* sp_aoc, const int* sp_atom, const int* sp_diag, const int* pair_list,
+              int npairs, int natom, const double* atom_xyz, const double* atom_z,
+              int64_t ncart, const double* D, const double* W, double* grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,16 @@ def _generate():
                     fh.write(text)
             # rough cost for scheduling: total angular momentum, fp64 first
             out.append((sum(c) * 2 + (prec == "f64"), path))
+        path = os.path.join(GEN, f"grad2e_{tag}.hip")
+        text = ('#include "grad_impl.h"\n'
+                "namespace qcmd {\n"
+                f"hipError_t grad2e_launch_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                f"  return grad2e_launch<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
+                "}\n}  // namespace qcmd\n")
+        if not os.path.exists(path) or open(path).read() != text:
+            with open(path, "w") as fh:
+                fh.write(text)
+        out.append((sum(c) * 2 + 1, path))
     return out
 
 

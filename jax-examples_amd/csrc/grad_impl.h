@@ -1,0 +1,377 @@
+// grad_impl.h -- two-electron part of the analytic nuclear gradient,
+// contracted on the fly (no derivative integrals are stored).
+//
+//   dE_2e/dX = 1/2 sum_{canonical shell quartets} f sum_{abcd} G_abcd d(ab|cd)/dX
+//   G_abcd   = Dj_ab Dj_cd - cx sum_s (Dk_s,ac Dk_s,bd + Dk_s,ad Dk_s,bc)
+// (RHF: Dj = Dk = D, cx = 1/4; UHF: Dj = Da + Db, Dk = {Da, Db}, cx = 1/2),
+// f = the number of distinct images of the shell quartet, all matrices in
+// the Cartesian AO basis.  Derivatives follow from the Gaussian identity
+// d/dA_x x_A^a e^{-alpha x_A^2} = 2 alpha x_A^{a+1} e^{..} - a x_A^{a-1} e^{..}
+// applied to the 1-D Hermite coefficients (E^A_t = 2 alpha E^{a+1,b}_t -
+// a E^{a-1,b}_t), for centres A, B and C; D by translational invariance.
+// This is the contraction the reference's rhf_gradient forms by finite
+// differences of full integral rebuilds (jax_qc/grad/rhf_grad.py:153-209).
+//
+// Layout: one 64-lane wave per contracted quartet, as eri_coop_kernel:
+// stage 1 builds R^0_tuv up to L+1 for the bra primitive pairs of a chunk
+// (levels in LDS); stage 2: lane (cd, g) owns ket pair cd and bra
+// primitives g, g+G, ...; 9 accumulators (A, B, C x 3) per lane, summed
+// over the wave at the end and added atomically into grad[atom][xyz].
+#pragma once
+#include "eri_coop.h"
+
+namespace qcmd {
+
+struct Grad2eArgs {
+  const double* pairs64;
+  const int* sp_prim_off;
+  const int* sp_nprim;
+  const int* sp_aoc;   // per shell pair: Cartesian AO offsets (shell 1, shell 2)
+  const int* sp_atom;  // per shell pair: atoms (shell 1, shell 2)
+  const int* sp_diag;  // per shell pair: 1 when both shells are the same
+  const int* bra_list;
+  const int* ket_list;
+  const int64_t* offsets;
+  const int* kstart;
+  int nbra;
+  int64_t nquartets;
+  int64_t ncart;
+  const double* dj;
+  const double* dk0;
+  const double* dk1;
+  int nk;
+  double cx;
+  double* grad;
+  const double* boys64;
+};
+
+constexpr int GMAXP = 9;
+
+template <int LA, int LB, int LC, int LD>
+struct GradShared {
+  static constexpr int LAB = LA + LB, LCD = LC + LD, L1 = LAB + LCD + 1;
+  static constexpr int EA = (LA + 2) * (LB + 2) * (LAB + 3);
+  static constexpr int EC = (LC + 2) * (LD + 2) * (LCD + 3);
+  static constexpr int NHL = nherm(L1);
+  static constexpr int NAB = ncart(LA) * ncart(LB), NCD = ncart(LC) * ncart(LD);
+  double eb[GMAXP][3][EA];
+  double bexp[GMAXP][2];
+  double ek[3][EC];
+  double kexp[2];
+  double f[GMAXP][L1 + 1];
+  double pq[GMAXP][4];
+  double r[2 * GMAXP * NHL > 64 * 9 ? 2 * GMAXP * NHL : 64 * 9];
+  double gam[NAB * NCD];
+};
+
+// E[i][j][t] in a table built for (i <= LA+1, j <= LB+1, t <= LA+LB+2).
+template <int LA, int LB>
+__device__ __forceinline__ double ex(const double* e, int i, int j, int t) {
+  if (i < 0 || j < 0) return 0.0;
+  return e[(i * (LB + 2) + j) * (LA + LB + 3) + t];
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void grad2e_coop_kernel(Grad2eArgs args) {
+  using S = GradShared<LA, LB, LC, LD>;
+  constexpr int LAB = S::LAB, L1 = S::L1;
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = S::NAB, NCD = S::NCD, NHL = S::NHL, EA = S::EA, EC = S::EC;
+  constexpr int G = 64 / NCD;
+  const HermTable<L1>& HT = kHermTable<L1>;
+  __shared__ S sh;
+
+  const int lane = threadIdx.x;
+  // quartet -> (bra pair, ket pair) as in the ERI kernels
+  int lo = 0, hi = args.nbra - 1;
+  const int64_t gid = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (args.offsets[mid] <= gid)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int bp = args.bra_list[lo];
+  const int kp = args.ket_list[args.kstart[lo] + (int)(gid - args.offsets[lo])];
+  const double* __restrict__ pp = args.pairs64;
+  const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
+  const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
+  const int64_t oa = args.sp_aoc[2 * bp], ob = args.sp_aoc[2 * bp + 1];
+  const int64_t oc = args.sp_aoc[2 * kp], od = args.sp_aoc[2 * kp + 1];
+  const int64_t nc = args.ncart;
+
+  // G_abcd for the whole block
+  for (int e = lane; e < NAB * NCD; e += 64) {
+    const int ab = e / NCD, cd = e % NCD;
+    const int64_t mu = oa + ab / NB, nu = ob + ab % NB, la = oc + cd / ND, si = od + cd % ND;
+    double g = args.dj[mu * nc + nu] * args.dj[la * nc + si];
+    double k = args.dk0[mu * nc + la] * args.dk0[nu * nc + si] +
+               args.dk0[mu * nc + si] * args.dk0[nu * nc + la];
+    if (args.nk > 1)
+      k += args.dk1[mu * nc + la] * args.dk1[nu * nc + si] +
+           args.dk1[mu * nc + si] * args.dk1[nu * nc + la];
+    sh.gam[e] = g - args.cx * k;
+  }
+
+  const int cd = lane % NCD, g = lane / NCD;
+  const bool active = g < G;
+  const int c = cd / ND, d = cd % ND;
+  const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
+  const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
+  const int sx = cx + dx, sy = cy + dy, sz = cz + dz;
+
+  double acc[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) acc[i] = 0.0;
+
+  for (int ib0 = 0; ib0 < npb; ib0 += GMAXP) {
+    const int nb = min(GMAXP, npb - ib0);
+    __syncthreads();
+    if (lane < nb) {
+      const double* P = pp + (int64_t)(pb0 + ib0 + lane) * PP_STRIDE;
+      const double oo2p = 0.5 / P[PP_P];
+      double E[LA + 2][LB + 2][LAB + 3];
+#pragma unroll
+      for (int dim = 0; dim < 3; ++dim) {
+        hermite_e<LA + 1, LB + 1, double>(E, P[PP_PAX + dim], P[PP_PBX + dim], oo2p);
+#pragma unroll
+        for (int i = 0; i <= LA + 1; ++i)
+#pragma unroll
+          for (int j = 0; j <= LB + 1; ++j)
+#pragma unroll
+            for (int t = 0; t <= LAB + 2; ++t)
+              sh.eb[lane][dim][(i * (LB + 2) + j) * (LAB + 3) + t] = E[i][j][t];
+      }
+      sh.bexp[lane][0] = P[PP_ALPHA];
+      sh.bexp[lane][1] = P[PP_P] - P[PP_ALPHA];
+    }
+    for (int ik = 0; ik < npk; ++ik) {
+      const double* Q = pp + (int64_t)(pk0 + ik) * PP_STRIDE;
+      const double q = Q[PP_P];
+      __syncthreads();
+      if (lane < 3) {
+        double E[LC + 2][LD + 2][S::LCD + 3];
+        hermite_e<LC + 1, LD + 1, double>(E, Q[PP_PAX + lane], Q[PP_PBX + lane], 0.5 / q);
+#pragma unroll
+        for (int i = 0; i <= LC + 1; ++i)
+#pragma unroll
+          for (int j = 0; j <= LD + 1; ++j)
+#pragma unroll
+            for (int t = 0; t <= S::LCD + 2; ++t)
+              sh.ek[lane][(i * (LD + 2) + j) * (S::LCD + 3) + t] = E[i][j][t];
+        if (lane == 0) {
+          sh.kexp[0] = Q[PP_ALPHA];
+          sh.kexp[1] = q - Q[PP_ALPHA];
+        }
+      }
+      if (lane >= 32 && lane - 32 < nb) {
+        const int ib = lane - 32;
+        const double* P = pp + (int64_t)(pb0 + ib0 + ib) * PP_STRIDE;
+        const double p = P[PP_P];
+        const double pqm = p * q, ps = p + q, alpha = pqm / ps;
+        const double X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
+                     Z = P[PP_PX + 2] - Q[PP_PX + 2];
+        double F[L1 + 1];
+        boys_fn<L1, double>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
+        double pw = 1.0;
+#pragma unroll
+        for (int n = 0; n <= L1; ++n) {
+          sh.f[ib][n] = pw * F[n];
+          pw *= -2.0 * alpha;
+        }
+        sh.pq[ib][0] = X;
+        sh.pq[ib][1] = Y;
+        sh.pq[ib][2] = Z;
+        sh.pq[ib][3] = 34.986836655249725693 / (pqm * sqrt(ps)) * P[PP_K] * Q[PP_K];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int n = L1; n >= 0; --n) {
+        const int nh = nherm(L1 - n);
+        double* cur = sh.r + (n & 1) * GMAXP * NHL;
+        const double* prv = sh.r + ((n + 1) & 1) * GMAXP * NHL;
+        for (int e = lane; e < nb * nh; e += 64) {
+          const int ib = e / nh, k = e - ib * nh;
+          double v;
+          if (k == 0) {
+            v = sh.f[ib][n];
+          } else {
+            const HermRec hr = HT.rec[k];
+            v = sh.pq[ib][hr.dir] * prv[ib * NHL + hr.src1];
+            if (hr.src2 >= 0) v += (double)hr.mult * prv[ib * NHL + hr.src2];
+          }
+          cur[ib * NHL + k] = v;
+        }
+        __syncthreads();
+      }
+      if (active) {
+        const double ga2 = 2.0 * sh.kexp[0];
+        const double* ekx = sh.ek[0];
+        const double* eky = sh.ek[1];
+        const double* ekz = sh.ek[2];
+        for (int ib = g; ib < nb; ib += G) {
+          const double* Rb = sh.r + ib * NHL;
+          const double al2 = 2.0 * sh.bexp[ib][0], be2 = 2.0 * sh.bexp[ib][1];
+          const double pref = sh.pq[ib][3];
+          const double* ebx = sh.eb[ib][0];
+          const double* eby = sh.eb[ib][1];
+          const double* ebz = sh.eb[ib][2];
+          // h_tuv, t+u+v <= LAB+1 (ket undifferentiated)
+          double h[nherm(LAB + 1)];
+#pragma unroll
+          for (int t = 0; t <= LAB + 1; ++t)
+#pragma unroll
+            for (int u = 0; u <= LAB + 1 - t; ++u)
+#pragma unroll
+              for (int v = 0; v <= LAB + 1 - t - u; ++v) {
+                double s = 0.0;
+                for (int tt = 0; tt <= sx; ++tt) {
+                  const double e1 = ex<LC, LD>(ekx, cx, dx, tt);
+                  for (int uu = 0; uu <= sy; ++uu) {
+                    const double e2 = e1 * ex<LC, LD>(eky, cy, dy, uu);
+                    for (int vv = 0; vv <= sz; ++vv) {
+                      const double e3 = e2 * ex<LC, LD>(ekz, cz, dz, vv);
+                      const double r = Rb[herm_index(t + tt, u + uu, v + vv)];
+                      s += ((tt + uu + vv) & 1) ? -e3 * r : e3 * r;
+                    }
+                  }
+                }
+                h[herm_index(t, u, v)] = s;
+              }
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              const double gm = sh.gam[(a * NB + b) * NCD + cd] * pref;
+              const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
+              const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
+              double sAx = 0, sAy = 0, sAz = 0, sBx = 0, sBy = 0, sBz = 0;
+#pragma unroll
+              for (int t = 0; t <= LAB + 1; ++t) {
+                if (t > ax + bx + 1) continue;
+                const double e0x = ex<LA, LB>(ebx, ax, bx, t);
+                const double eAx = al2 * ex<LA, LB>(ebx, ax + 1, bx, t) - ax * ex<LA, LB>(ebx, ax - 1, bx, t);
+                const double eBx = be2 * ex<LA, LB>(ebx, ax, bx + 1, t) - bx * ex<LA, LB>(ebx, ax, bx - 1, t);
+#pragma unroll
+                for (int u = 0; u <= LAB + 1; ++u) {
+                  if (u > ay + by + 1 || t + u > LAB + 1) continue;
+                  const double e0y = ex<LA, LB>(eby, ay, by, u);
+                  const double eAy = al2 * ex<LA, LB>(eby, ay + 1, by, u) - ay * ex<LA, LB>(eby, ay - 1, by, u);
+                  const double eBy = be2 * ex<LA, LB>(eby, ay, by + 1, u) - by * ex<LA, LB>(eby, ay, by - 1, u);
+#pragma unroll
+                  for (int v = 0; v <= LAB + 1; ++v) {
+                    if (v > az + bz + 1 || t + u + v > LAB + 1) continue;
+                    const double e0z = ex<LA, LB>(ebz, az, bz, v);
+                    const double eAz = al2 * ex<LA, LB>(ebz, az + 1, bz, v) - az * ex<LA, LB>(ebz, az - 1, bz, v);
+                    const double eBz = be2 * ex<LA, LB>(ebz, az, bz + 1, v) - bz * ex<LA, LB>(ebz, az, bz - 1, v);
+                    const double hv = h[herm_index(t, u, v)];
+                    sAx += eAx * e0y * e0z * hv;
+                    sAy += e0x * eAy * e0z * hv;
+                    sAz += e0x * e0y * eAz * hv;
+                    sBx += eBx * e0y * e0z * hv;
+                    sBy += e0x * eBy * e0z * hv;
+                    sBz += e0x * e0y * eBz * hv;
+                  }
+                }
+              }
+              acc[0] += gm * sAx;
+              acc[1] += gm * sAy;
+              acc[2] += gm * sAz;
+              acc[3] += gm * sBx;
+              acc[4] += gm * sBy;
+              acc[5] += gm * sBz;
+            }
+          // C derivatives, one Cartesian direction at a time
+#pragma unroll
+          for (int dim = 0; dim < 3; ++dim) {
+            double hc[nherm(LAB)];
+            const int ex2 = sx + (dim == 0), ey2 = sy + (dim == 1), ez2 = sz + (dim == 2);
+#pragma unroll
+            for (int t = 0; t <= LAB; ++t)
+#pragma unroll
+              for (int u = 0; u <= LAB - t; ++u)
+#pragma unroll
+                for (int v = 0; v <= LAB - t - u; ++v) {
+                  double s = 0.0;
+                  for (int tt = 0; tt <= ex2; ++tt) {
+                    const double e1 = dim == 0 ? ga2 * ex<LC, LD>(ekx, cx + 1, dx, tt) -
+                                                     cx * ex<LC, LD>(ekx, cx - 1, dx, tt)
+                                               : ex<LC, LD>(ekx, cx, dx, tt);
+                    for (int uu = 0; uu <= ey2; ++uu) {
+                      const double e2 = e1 * (dim == 1 ? ga2 * ex<LC, LD>(eky, cy + 1, dy, uu) -
+                                                             cy * ex<LC, LD>(eky, cy - 1, dy, uu)
+                                                       : ex<LC, LD>(eky, cy, dy, uu));
+                      for (int vv = 0; vv <= ez2; ++vv) {
+                        const double e3 = e2 * (dim == 2 ? ga2 * ex<LC, LD>(ekz, cz + 1, dz, vv) -
+                                                               cz * ex<LC, LD>(ekz, cz - 1, dz, vv)
+                                                         : ex<LC, LD>(ekz, cz, dz, vv));
+                        const double r = Rb[herm_index(t + tt, u + uu, v + vv)];
+                        s += ((tt + uu + vv) & 1) ? -e3 * r : e3 * r;
+                      }
+                    }
+                  }
+                  hc[herm_index(t, u, v)] = s;
+                }
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+              for (int b = 0; b < NB; ++b) {
+                const double gm = sh.gam[(a * NB + b) * NCD + cd] * pref;
+                const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
+                const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
+                double sC = 0.0;
+#pragma unroll
+                for (int t = 0; t <= LAB; ++t) {
+                  if (t > ax + bx) continue;
+#pragma unroll
+                  for (int u = 0; u <= LAB; ++u) {
+                    if (u > ay + by) continue;
+                    const double exy = ex<LA, LB>(ebx, ax, bx, t) * ex<LA, LB>(eby, ay, by, u);
+#pragma unroll
+                    for (int v = 0; v <= LAB; ++v) {
+                      if (v > az + bz) continue;
+                      sC += exy * ex<LA, LB>(ebz, az, bz, v) * hc[herm_index(t, u, v)];
+                    }
+                  }
+                }
+                acc[6 + dim] += gm * sC;
+              }
+          }
+        }
+      }
+    }
+  }
+  (void)EA;
+  (void)EC;
+  (void)NC;
+  // wave sum of the 9 accumulators
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    for (int w = 32; w > 0; w >>= 1) acc[i] += __shfl_xor(acc[i], w);
+  if (lane == 0) {
+    const double f = (args.sp_diag[bp] ? 1.0 : 2.0) * (args.sp_diag[kp] ? 1.0 : 2.0) *
+                     (bp == kp ? 1.0 : 2.0) * 0.5;
+    const int A = args.sp_atom[2 * bp], B = args.sp_atom[2 * bp + 1];
+    const int C = args.sp_atom[2 * kp], D = args.sp_atom[2 * kp + 1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double gA = f * acc[k], gB = f * acc[3 + k], gC = f * acc[6 + k];
+      atomicAdd(&args.grad[3 * A + k], gA);
+      atomicAdd(&args.grad[3 * B + k], gB);
+      atomicAdd(&args.grad[3 * C + k], gC);
+      atomicAdd(&args.grad[3 * D + k], -(gA + gB + gC));
+    }
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t grad2e_launch(const Grad2eArgs& a, hipStream_t s) {
+  if (a.nquartets <= 0) return hipSuccess;
+  if (a.nquartets > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((grad2e_coop_kernel<LA, LB, LC, LD>), dim3((unsigned)a.nquartets), dim3(64), 0,
+                     s, a);
+  return hipGetLastError();
+}
+
+}  // namespace qcmd

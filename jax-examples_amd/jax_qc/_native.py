@@ -36,6 +36,19 @@ class QcEriArgs(ctypes.Structure):
     ]
 
 
+class QcGrad2eArgs(ctypes.Structure):
+    """Mirror of qc_grad2e_args (include/qcmd.h)."""
+
+    _fields_ = [
+        ("pairs64", c_void_p), ("sp_prim_off", c_void_p), ("sp_nprim", c_void_p),
+        ("sp_aoc", c_void_p), ("sp_atom", c_void_p), ("sp_diag", c_void_p),
+        ("bra_list", c_void_p), ("ket_list", c_void_p), ("offsets", c_void_p),
+        ("kstart", c_void_p), ("nbra", c_int), ("nquartets", c_i64), ("ncart", c_i64),
+        ("dj", c_void_p), ("dk0", c_void_p), ("dk1", c_void_p), ("nk", c_int),
+        ("cx", c_double), ("grad", c_void_p),
+    ]
+
+
 _SIGS = {
     "qc_init": [c_int],
     "qc_boys": [c_int, c_void_p, c_void_p, c_i64, c_void_p],
@@ -55,6 +68,10 @@ _SIGS = {
     "qc_pack_density": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
     "qc_unpack_fock": [c_void_p, c_void_p, c_double, c_i64, c_int, c_void_p, c_void_p],
     "qc_dense_jk": [c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
+    "qc_grad2e": [c_int, c_int, ctypes.POINTER(QcGrad2eArgs), c_void_p],
+    "qc_grad1e": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                  c_int, c_int, c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p,
+                  c_void_p],
 }
 _OPTIONAL_SIGS = {}
 

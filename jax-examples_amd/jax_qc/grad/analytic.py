@@ -1,0 +1,114 @@
+"""Analytic nuclear gradients from the device kernels.
+
+dE/dR = dE_nuc/dR + sum D d(T+V)/dR - sum W dS/dR + 1/2 sum G d(ERI)/dR,
+the Hellmann-Feynman + Pulay expression the reference evaluates in
+jax_qc/grad/rhf_grad.py:186-209 -- there with every derivative a central
+finite difference of full integral rebuilds at fixed D and W; here each
+term is contracted analytically on the GPU (qc_grad1e, qc_grad2e) without
+forming derivative integrals.  Valid for converged RHF and UHF states
+(for UHF: D = Da + Db, W = Wa + Wb, and the exchange term uses each spin
+density -- the reference applies its RHF expression to UHF results, which
+is not the UHF gradient).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from jax_qc import _native as nat
+from jax_qc.integrals import engine as eng
+
+
+def energy_weighted_density(C, orbital_energies, n_occ: int, factor: float = 2.0):
+    """W = factor * sum_{i < n_occ} eps_i C_i C_i^T (rhf_grad.py:92-103)."""
+    Co = C[:, :n_occ]
+    return factor * (Co * orbital_energies[:n_occ][None, :]) @ Co.T
+
+
+def nuclear_repulsion_gradient(mol) -> np.ndarray:
+    c = np.asarray(mol.coords, dtype=np.float64)
+    Z = np.asarray(mol.atomic_numbers, dtype=np.float64)
+    d = c[:, None, :] - c[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    np.fill_diagonal(r, np.inf)
+    return -np.sum((Z[:, None] * Z[None, :] / r ** 3)[:, :, None] * d, axis=1)
+
+
+def _densities(mol, res):
+    """(Dj, [Dk...], cx, W) in the spherical AO basis."""
+    if res.uhf is not None:
+        u = res.uhf
+        Da, Db = u.density_alpha, u.density_beta
+        Wa = energy_weighted_density(u.coefficients_alpha, u.orbital_energies_alpha, mol.n_alpha, 1.0)
+        Wb = energy_weighted_density(u.coefficients_beta, u.orbital_energies_beta, mol.n_beta, 1.0)
+        return Da + Db, [Da, Db], 0.5, Wa + Wb
+    D = res.state.density
+    W = energy_weighted_density(res.state.coefficients, res.state.orbital_energies,
+                                mol.n_electrons // 2, 2.0)
+    return D, [D], 0.25, W
+
+
+def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_table=None,
+                        parts=("1e", "2e")) -> torch.Tensor:
+    """sum D d(T+V) - sum W dS + 1/2 sum G dERI on the device, (natom, 3)."""
+    dev = nat.device()
+    nat.ensure_init(dev)
+    pt = pair_table or eng.PairTable(basis, dev)
+    Cm = torch.from_numpy(eng.cart_to_sph_matrix(basis)).to(dev)
+
+    def cart(M):
+        return (Cm.T @ M @ Cm).contiguous()
+
+    Djc = cart(Dj)
+    Dkc = [cart(x) for x in Dk]
+    Wc = cart(W)
+    nat_ = len(mol.atomic_numbers)
+    grad = torch.zeros(nat_ * 3, dtype=torch.float64, device=dev)
+    s = nat.stream_ptr()
+    if "1e" in parts:
+        xyz = torch.as_tensor(np.ascontiguousarray(mol.coords, dtype=np.float64), device=dev)
+        zz = torch.as_tensor(np.asarray(mol.atomic_numbers, dtype=np.float64), device=dev)
+        for c in range(6):
+            lst = pt.class_pairs[c]
+            if len(lst) == 0:
+                continue
+            ld = torch.from_numpy(lst).to(dev)
+            nat.call("qc_grad1e", c, nat.ptr(pt.pairs64), nat.ptr(pt.sp_prim_off),
+                     nat.ptr(pt.sp_nprim), nat.ptr(pt.sp_aoc), nat.ptr(pt.sp_atom),
+                     nat.ptr(pt.sp_diag), nat.ptr(ld), len(lst), nat_, nat.ptr(xyz), nat.ptr(zz),
+                     pt.ncart, nat.ptr(Djc), nat.ptr(Wc), nat.ptr(grad), s)
+    if "2e" in parts:
+        plan = eng.plan_quartets(pt, tau)
+        for L in plan.launches:
+            a = nat.QcGrad2eArgs()
+            a.pairs64 = pt.pairs64.data_ptr()
+            a.sp_prim_off = pt.sp_prim_off.data_ptr()
+            a.sp_nprim = pt.sp_nprim.data_ptr()
+            a.sp_aoc = pt.sp_aoc.data_ptr()
+            a.sp_atom = pt.sp_atom.data_ptr()
+            a.sp_diag = pt.sp_diag.data_ptr()
+            a.bra_list = L.bra.data_ptr()
+            a.ket_list = L.ket.data_ptr()
+            a.offsets = L.offsets.data_ptr()
+            a.kstart = L.kstart.data_ptr()
+            a.nbra = L.nbra
+            a.nquartets = L.nquartets
+            a.ncart = pt.ncart
+            a.dj = Djc.data_ptr()
+            a.dk0 = Dkc[0].data_ptr()
+            a.dk1 = Dkc[1].data_ptr() if len(Dkc) > 1 else 0
+            a.nk = len(Dkc)
+            a.cx = cx
+            a.grad = grad.data_ptr()
+            nat.call("qc_grad2e", L.cab, L.ccd, ctypes.byref(a), s)
+    return grad.reshape(nat_, 3)
+
+
+def scf_gradient(mol, basis, res, tau: float = 1e-13, pair_table=None) -> torch.Tensor:
+    """Full nuclear gradient (Eh/Bohr) of a converged RHF or UHF result."""
+    Dj, Dk, cx, W = _densities(mol, res)
+    g = electronic_gradient(mol, basis, Dj, Dk, cx, W, tau, pair_table)
+    return g + torch.from_numpy(nuclear_repulsion_gradient(mol)).to(g.device)

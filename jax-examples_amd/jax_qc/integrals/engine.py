@@ -66,6 +66,12 @@ class PairTable:
         self.dev = dev or nat.device()
         L, npr, off, ex, co, ce, ao = (sa[k] for k in ("l", "nprim", "prim_off", "exps", "coefs",
                                                         "centers", "ao_off"))
+        ncl = (L + 1) * (L + 2) // 2
+        aoc = np.zeros(self.nshell + 1, dtype=np.int64)
+        aoc[1:] = np.cumsum(ncl)
+        self.ncart = int(aoc[-1])
+        self.cart_off = aoc
+        atom = sa["atom"]
         pairs, sp_off, sp_n, sp_ao, sp_cls, sp_sh = [], [], [], [], [], []
         npp = 0
         for i in range(self.nshell):
@@ -110,6 +116,12 @@ class PairTable:
         self.sp_prim_off = torch.tensor(sp_off, dtype=torch.int32, device=d)
         self.sp_nprim = torch.tensor(sp_n, dtype=torch.int32, device=d)
         self.sp_ao = torch.from_numpy(self.sp_ao_host.copy()).to(d)
+        a_, b_ = self.sp_shells[:, 0], self.sp_shells[:, 1]
+        self.sp_aoc = torch.from_numpy(
+            np.stack([aoc[a_], aoc[b_]], 1).astype(np.int32).reshape(-1)).to(d)
+        self.sp_atom = torch.from_numpy(
+            np.stack([atom[a_], atom[b_]], 1).astype(np.int32).reshape(-1)).to(d)
+        self.sp_diag = torch.from_numpy((a_ == b_).astype(np.int32)).to(d)
         self.class_pairs: Dict[int, np.ndarray] = {
             c: np.nonzero(self.sp_class == c)[0].astype(np.int32) for c in range(6)}
         self.Q = None  # per shell pair Schwarz bound (host numpy)
@@ -290,6 +302,27 @@ def plan_flops(pt: PairTable, plan: QuartetPlan, nprim_quartet: float | None = N
         lc, ld = CLASS_L[ccd]
         out[fp32] += n * mean_np[cab] * mean_np[ccd] * flops_per_primitive_quartet(la, lb, lc, ld)
     return out
+
+
+def cart_to_sph_matrix(basis) -> np.ndarray:
+    """Block-diagonal (n_sph x n_cart) matrix C with rows in the libcint
+    spherical order (jax_qc/integrals/spherical.py:34-67): X_sph = C X_cart C^T
+    and D_cart = C^T D_sph C."""
+    r3 = 3.0 ** 0.5
+    blocks = {0: np.eye(1), 1: np.eye(3),
+              2: np.array([[0, r3, 0, 0, 0, 0], [0, 0, 0, 0, r3, 0], [-0.5, 0, 0, -0.5, 0, 1.0],
+                           [0, 0, r3, 0, 0, 0], [0.5 * r3, 0, 0, -0.5 * r3, 0, 0]])}
+    ls = [s.angular_momentum for s in basis.shells]
+    ns = sum(2 * l + 1 for l in ls)
+    nc = sum((l + 1) * (l + 2) // 2 for l in ls)
+    C = np.zeros((ns, nc))
+    i = j = 0
+    for l in ls:
+        b = blocks[l]
+        C[i:i + b.shape[0], j:j + b.shape[1]] = b
+        i += b.shape[0]
+        j += b.shape[1]
+    return C
 
 
 def sm_tiles_per_dim(nbf: int) -> int:
