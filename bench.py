@@ -130,13 +130,19 @@ def _path(mol, bs, mode, scf, tau_f32_tile):
         store.use_compact(tau_f32=tau_f32_tile)
     torch.cuda.synchronize()
     t_ints = time.perf_counter() - t0
-    res, t_scf = None, 0.0
+    res, t_scf, grad, t_grad = None, 0.0, None, 0.0
     if scf:
+        from jax_qc.grad import scf_gradient
+
         t1 = time.perf_counter()
         res = jax_qc.run_scf(mol, ints, CalcConfig(method="rhf", scf_conv=1e-10))
         torch.cuda.synchronize()
         t_scf = time.perf_counter() - t1
-    return ints, store, store.P, res, t_ints, t_scf
+        t2 = time.perf_counter()
+        grad = scf_gradient(mol, bs, res, tau=1e-12, pair_table=store.pt)
+        torch.cuda.synchronize()
+        t_grad = time.perf_counter() - t2
+    return ints, store, store.P, res, t_ints, t_scf, grad, t_grad
 
 
 def main():
@@ -187,7 +193,8 @@ def main():
         del src, dst
     results = {}
     for mode in modes:
-        ints, store, Mx, res, t_ints, t_scf = _path(mol, bs, mode, scf, args.tile_f32)
+        ints, store, Mx, res, t_ints, t_scf, grad, t_grad = _path(mol, bs, mode, scf,
+                                                                  args.tile_f32)
         D = (res.state.density.contiguous() if res is not None
              else torch.eye(n, dtype=torch.float64, device=dev) * 0.1)
         rate, dt, avg_ms = _fock_rate(store, Mx, D, ints.H_core.contiguous(), n, args.steps,
@@ -197,16 +204,19 @@ def main():
                  eri_s=store.eri_seconds, n64=store.plan.n64, n32=store.plan.n32,
                  nscr=store.plan.n_screened)
         if res is not None:
-            r.update(energy=res.energy, iters=res.n_iterations, converged=res.converged)
+            r.update(energy=res.energy, iters=res.n_iterations, converged=res.converged,
+                     t_grad=t_grad, energy_grad_wall_s=t_ints + t_scf + t_grad)
+            r["_grad"] = grad.cpu().numpy()
         if isinstance(Mx, CompactSupermatrix):
             r.update(tiles_zero=Mx.n_zero, tiles_f32=Mx.n_f32, tiles_f64=Mx.n_f64)
         results[mode] = r
         del ints, store, Mx
         torch.cuda.empty_cache()
 
-    dE = None
+    dE = dG = None
     if "fp64" in results and "mixed" in results and scf:
         dE = abs(results["mixed"]["energy"] - results["fp64"]["energy"])
+        dG = float(np.abs(results["mixed"]["_grad"] - results["fp64"]["_grad"]).max())
     if args.mode == "auto":
         pick = "mixed" if (dE is not None and dE <= 1e-6) else "fp64"
     else:
@@ -216,10 +226,11 @@ def main():
     cfg = {"workload": f"(H2O)_{int(np.prod(grid))} even-tempered RHF Fock build",
            "grid": list(grid), "seed": 0, "n_basis": n, "n_shells": len(bs.shells),
            "schwarz_tau": 1e-12, "path": pick, "dE_mixed_vs_fp64": dE,
+           "max_dgrad_mixed_vs_fp64": dG,
            "tile_f32_tau": args.tile_f32, "eri_fp32_tau": 1e-4}
     for m, r in results.items():
         for k, v in r.items():
-            if k in ("dt",):
+            if k in ("dt", "_grad"):
                 continue
             cfg[f"{m}_{k}"] = round(v, 6) if isinstance(v, float) else v
     out = {
