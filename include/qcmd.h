@@ -145,6 +145,8 @@ typedef struct {
   int nk;
   double cx;
   double* grad;        /* natom x 3, accumulated (atomic adds) */
+  int variant;         /* 0 library default, 1 one thread per quartet (low
+                          classes only), 2 one wave per quartet */
 } qc_grad2e_args;
 
 /* 1/2 sum_quartets f sum_abcd G_abcd d(ab|cd)/dR for one class pair,

@@ -31,6 +31,10 @@ ERI_CLASSES = [
     (2, 1, 1, 1), (2, 1, 2, 0), (2, 1, 2, 1), (2, 2, 0, 0), (2, 2, 1, 0), (2, 2, 1, 1),
     (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2),
 ]
+# derivative-ERI classes that also get the one-thread-per-quartet kernel
+# (the rest have only the one-wave-per-quartet form: their G blocks and
+# Hermite tables do not fit a thread's registers)
+GRAD_PER_THREAD = {"0000", "1000", "1010", "1100", "1110", "2000", "2010", "2100"}
 STATIC_SOURCES = ["capi.hip", "oneint.hip", "jk.hip", "grad.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -70,7 +74,11 @@ def _generate():
                 "namespace qcmd {\n"
                 f"hipError_t grad2e_launch_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 f"  return grad2e_launch<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
-                "}\n}  // namespace qcmd\n")
+                "}\n"
+                f"hipError_t grad2e_launcht_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
+                   if tag in GRAD_PER_THREAD else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
+                + "}\n}  // namespace qcmd\n")
         if not os.path.exists(path) or open(path).read() != text:
             with open(path, "w") as fh:
                 fh.write(text)

@@ -84,7 +84,7 @@ const float* boys_table32() {
 template <int N, typename Real>
 __global__ void boys_kernel(const Real* __restrict__ t, Real* __restrict__ F, int64_t n,
                             const Real* __restrict__ tab) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t i = flat_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Real f[N + 1];
   boys_fn<N, Real>(t[i], tab, f);
@@ -96,7 +96,7 @@ template <typename Real>
 static int launch_boys(int nmax, const Real* t, Real* F, int64_t n, hipStream_t s,
                        const Real* tab) {
   if (n <= 0) return 0;
-  dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  dim3 grid = grid_1d((n + 255) / 256, 256), block(256);
   switch (nmax) {
 #define QC_B(N) \
   case N: hipLaunchKernelGGL((boys_kernel<N, Real>), grid, block, 0, s, t, F, n, tab); break;

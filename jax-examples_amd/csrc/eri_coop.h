@@ -95,7 +95,9 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
 
   const int lane = threadIdx.x;
   int bp, kp;
-  locate(args, blockIdx.x, bp, kp);
+  const int64_t gid = flat_block();
+  if (gid >= args.nquartets) return;  // whole work-group (one wave)
+  locate(args, gid, bp, kp);
   const Real* __restrict__ pp = pair_table<Real>(args);
   const Real* __restrict__ tab = boys_table<Real>(args);
   const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
@@ -327,7 +329,7 @@ template <int LA, int LB, int LC, int LD, typename Real>
 hipError_t eri_launch_coop(const EriArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   if (a.nquartets > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((eri_coop_kernel<LA, LB, LC, LD, Real>), dim3((unsigned)a.nquartets),
+  hipLaunchKernelGGL((eri_coop_kernel<LA, LB, LC, LD, Real>), grid_1d(a.nquartets, 64),
                      dim3(64), 0, s, a);
   return hipGetLastError();
 }

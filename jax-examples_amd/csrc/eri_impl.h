@@ -73,7 +73,7 @@ template <int LA, int LB, int LC, int LD, typename Real>
 __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
   constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t gid = flat_block() * blockDim.x + threadIdx.x;
   if (gid >= args.nquartets) return;
   int bp, kp;
   locate(args, gid, bp, kp);
@@ -274,7 +274,7 @@ template <int LA, int LB, int LC, int LD, typename Real>
 hipError_t eri_launch_class(const EriArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   const int block = 128;
-  const dim3 grid((unsigned)((a.nquartets + block - 1) / block));
+  const dim3 grid = grid_1d((a.nquartets + block - 1) / block, block);
   hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, Real>), grid, dim3(block), 0, s, a);
   return hipGetLastError();
 }

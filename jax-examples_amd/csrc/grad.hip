@@ -175,7 +175,9 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
   }
 }
 
-#define QC_GDECL(A, B, C, D) hipError_t grad2e_launch_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
+#define QC_GDECL(A, B, C, D)                                                     \
+  hipError_t grad2e_launch_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);     \
+  hipError_t grad2e_launcht_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
 QC_ERI_CLASSES(QC_GDECL)
 #undef QC_GDECL
 
@@ -239,12 +241,19 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   a.grad = in->grad;
   a.boys64 = boys_table64();
   if (!a.boys64) return set_error("qc_grad2e: qc_init not called on the current device");
+  if (in->variant < 0 || in->variant > 2) return set_error("qc_grad2e: variant %d", in->variant);
+  // default: one thread per quartet up to (ps|ss)-size blocks, where the
+  // wave form leaves most lanes idle
+  const bool thread = in->variant == 1 || (in->variant == 0 && cab <= 3 && ccd <= 1);
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
-#define QC_GC(A, B, C, D) \
-  if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D)) e = grad2e_launch_##A##B##C##D(a, s);
+#define QC_GC(A, B, C, D)                                                  \
+  if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D)) \
+    e = thread ? grad2e_launcht_##A##B##C##D(a, s) : grad2e_launch_##A##B##C##D(a, s);
   QC_ERI_CLASSES(QC_GC)
 #undef QC_GC
+  if (e == hipErrorNotSupported)
+    return set_error("qc_grad2e: class (%d, %d) has no one-thread-per-quartet kernel", cab, ccd);
   return check(e, "qc_grad2e launch");
 }
 

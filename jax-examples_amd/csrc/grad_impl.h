@@ -84,7 +84,8 @@ __global__ __launch_bounds__(64) void grad2e_coop_kernel(Grad2eArgs args) {
   const int lane = threadIdx.x;
   // quartet -> (bra pair, ket pair) as in the ERI kernels
   int lo = 0, hi = args.nbra - 1;
-  const int64_t gid = blockIdx.x;
+  const int64_t gid = flat_block();
+  if (gid >= args.nquartets) return;  // whole work-group (one wave)
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (args.offsets[mid] <= gid)
@@ -365,11 +366,219 @@ __global__ __launch_bounds__(64) void grad2e_coop_kernel(Grad2eArgs args) {
   }
 }
 
+// One thread per contracted quartet, for the low classes (the G block and
+// the 9 accumulators stay in registers).  Per primitive quartet: E tables up
+// to (la+1, lb+1) and their centre derivatives 2 alpha E^{a+1} - a E^{a-1},
+// R up to L+1; for each ket Cartesian pair cd the ket-contracted Hermite
+// integrals h0_tuv (t+u+v <= la+lb+1) and, per direction, hC_tuv with the
+// ket's C-derivative coefficients (t+u+v <= la+lb); each bra pair ab then
+// takes its A-, B- and undifferentiated E against them.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
+  constexpr int LAB = LA + LB, LCD = LC + LD, L1 = LAB + LCD + 1;
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  constexpr int NAB = NA * NB, NCD = NC * ND;
+  constexpr int NH1 = nherm(LAB + 1), NH0 = nherm(LAB);
+  const int64_t gid = flat_block() * blockDim.x + threadIdx.x;
+  if (gid >= args.nquartets) return;
+  int lo = 0, hi = args.nbra - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (args.offsets[mid] <= gid)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int bp = args.bra_list[lo];
+  const int kp = args.ket_list[args.kstart[lo] + (int)(gid - args.offsets[lo])];
+  const double* __restrict__ pp = args.pairs64;
+  const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
+  const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
+  const int64_t oa = args.sp_aoc[2 * bp], ob = args.sp_aoc[2 * bp + 1];
+  const int64_t oc = args.sp_aoc[2 * kp], od = args.sp_aoc[2 * kp + 1];
+  const int64_t nc = args.ncart;
+
+  double gam[NAB][NCD];
+#pragma unroll
+  for (int ab = 0; ab < NAB; ++ab)
+#pragma unroll
+    for (int cd = 0; cd < NCD; ++cd) {
+      const int64_t mu = oa + ab / NB, nu = ob + ab % NB, la = oc + cd / ND, si = od + cd % ND;
+      double k = args.dk0[mu * nc + la] * args.dk0[nu * nc + si] +
+                 args.dk0[mu * nc + si] * args.dk0[nu * nc + la];
+      if (args.nk > 1)
+        k += args.dk1[mu * nc + la] * args.dk1[nu * nc + si] +
+             args.dk1[mu * nc + si] * args.dk1[nu * nc + la];
+      gam[ab][cd] = args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k;
+    }
+
+  double acc[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) acc[i] = 0.0;
+
+  for (int ik = 0; ik < npk; ++ik) {
+    const double* Q = pp + (int64_t)(pk0 + ik) * PP_STRIDE;
+    const double q = Q[PP_P], ga2 = 2.0 * Q[PP_ALPHA];
+    double Ek[3][LC + 2][LD + 2][LCD + 3];
+    double dEk[3][LC + 1][LD + 1][LCD + 2];
+#pragma unroll
+    for (int dim = 0; dim < 3; ++dim) {
+      hermite_e<LC + 1, LD + 1, double>(Ek[dim], Q[PP_PAX + dim], Q[PP_PBX + dim], 0.5 / q);
+#pragma unroll
+      for (int i = 0; i <= LC; ++i)
+#pragma unroll
+        for (int j = 0; j <= LD; ++j)
+#pragma unroll
+          for (int t = 0; t <= LCD + 1; ++t) {
+            double v = ga2 * Ek[dim][i + 1][j][t];
+            if (i > 0) v -= (double)i * Ek[dim][i > 0 ? i - 1 : 0][j][t];
+            dEk[dim][i][j][t] = v;
+          }
+    }
+    for (int ib = 0; ib < npb; ++ib) {
+      const double* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
+      const double p = P[PP_P], al2 = 2.0 * P[PP_ALPHA], be2 = 2.0 * (p - P[PP_ALPHA]);
+      double Eb[3][LA + 2][LB + 2][LAB + 3];
+      double dEa[3][LA + 1][LB + 1][LAB + 2], dEb[3][LA + 1][LB + 1][LAB + 2];
+#pragma unroll
+      for (int dim = 0; dim < 3; ++dim) {
+        hermite_e<LA + 1, LB + 1, double>(Eb[dim], P[PP_PAX + dim], P[PP_PBX + dim], 0.5 / p);
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+          for (int j = 0; j <= LB; ++j)
+#pragma unroll
+            for (int t = 0; t <= LAB + 1; ++t) {
+              double va = al2 * Eb[dim][i + 1][j][t];
+              if (i > 0) va -= (double)i * Eb[dim][i > 0 ? i - 1 : 0][j][t];
+              double vb = be2 * Eb[dim][i][j + 1][t];
+              if (j > 0) vb -= (double)j * Eb[dim][i][j > 0 ? j - 1 : 0][t];
+              dEa[dim][i][j][t] = va;
+              dEb[dim][i][j][t] = vb;
+            }
+      }
+      const double pqm = p * q, ps = p + q, alpha = pqm / ps;
+      const double X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
+                   Z = P[PP_PX + 2] - Q[PP_PX + 2];
+      double F[L1 + 1];
+      boys_fn<L1, double>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
+      double R[L1 + 1][L1 + 1][L1 + 1];
+      hermite_r<L1, double>(R, F, -2.0 * alpha, X, Y, Z);
+      const double pref = 34.986836655249725693 / (pqm * sqrt(ps)) * P[PP_K] * Q[PP_K];
+
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          const int kx = CX[LC][c] + 0, ky = CY[LC][c], kz = CZ[LC][c];
+          const int jx = CX[LD][d], jy = CY[LD][d], jz = CZ[LD][d];
+          double h0[NH1], hC[3][NH0];
+#pragma unroll
+          for (int t = 0; t <= LAB + 1; ++t)
+#pragma unroll
+            for (int u = 0; u <= LAB + 1 - t; ++u)
+#pragma unroll
+              for (int v = 0; v <= LAB + 1 - t - u; ++v) {
+                const bool low = t + u + v <= LAB;
+                double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+                for (int tt = 0; tt <= kx + jx + 1; ++tt)
+#pragma unroll
+                  for (int uu = 0; uu <= ky + jy + 1; ++uu)
+#pragma unroll
+                    for (int vv = 0; vv <= kz + jz + 1; ++vv) {
+                      const int over = (tt > kx + jx) + (uu > ky + jy) + (vv > kz + jz);
+                      if (over > 1) continue;
+                      double r = R[t + tt][u + uu][v + vv];
+                      if ((tt + uu + vv) & 1) r = -r;
+                      const double ex0 = Ek[0][kx][jx][tt], ey0 = Ek[1][ky][jy][uu],
+                                   ez0 = Ek[2][kz][jz][vv];
+                      if (over == 0) s0 += ex0 * ey0 * ez0 * r;
+                      if (low) {
+                        if (uu <= ky + jy && vv <= kz + jz) sx += dEk[0][kx][jx][tt] * ey0 * ez0 * r;
+                        if (tt <= kx + jx && vv <= kz + jz) sy += ex0 * dEk[1][ky][jy][uu] * ez0 * r;
+                        if (tt <= kx + jx && uu <= ky + jy) sz += ex0 * ey0 * dEk[2][kz][jz][vv] * r;
+                      }
+                    }
+                h0[herm_index(t, u, v)] = s0;
+                if (low) {
+                  hC[0][herm_index(t, u, v)] = sx;
+                  hC[1][herm_index(t, u, v)] = sy;
+                  hC[2][herm_index(t, u, v)] = sz;
+                }
+              }
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
+              const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
+              double s[9];
+#pragma unroll
+              for (int i = 0; i < 9; ++i) s[i] = 0.0;
+#pragma unroll
+              for (int t = 0; t <= ax + bx + 1; ++t)
+#pragma unroll
+                for (int u = 0; u <= ay + by + 1; ++u)
+#pragma unroll
+                  for (int v = 0; v <= az + bz + 1; ++v) {
+                    const int over = (t > ax + bx) + (u > ay + by) + (v > az + bz);
+                    if (over > 1) continue;
+                    const double e0x = Eb[0][ax][bx][t], e0y = Eb[1][ay][by][u], e0z = Eb[2][az][bz][v];
+                    const double hv = h0[herm_index(t, u, v)];
+                    if (u <= ay + by && v <= az + bz) {
+                      s[0] += dEa[0][ax][bx][t] * e0y * e0z * hv;
+                      s[3] += dEb[0][ax][bx][t] * e0y * e0z * hv;
+                    }
+                    if (t <= ax + bx && v <= az + bz) {
+                      s[1] += e0x * dEa[1][ay][by][u] * e0z * hv;
+                      s[4] += e0x * dEb[1][ay][by][u] * e0z * hv;
+                    }
+                    if (t <= ax + bx && u <= ay + by) {
+                      s[2] += e0x * e0y * dEa[2][az][bz][v] * hv;
+                      s[5] += e0x * e0y * dEb[2][az][bz][v] * hv;
+                    }
+                    if (over == 0) {
+                      const double e0 = e0x * e0y * e0z;
+                      s[6] += e0 * hC[0][herm_index(t, u, v)];
+                      s[7] += e0 * hC[1][herm_index(t, u, v)];
+                      s[8] += e0 * hC[2][herm_index(t, u, v)];
+                    }
+                  }
+              const double gm = gam[a * NB + b][c * ND + d] * pref;
+#pragma unroll
+              for (int i = 0; i < 9; ++i) acc[i] += gm * s[i];
+            }
+        }
+    }
+  }
+  const double f = (args.sp_diag[bp] ? 1.0 : 2.0) * (args.sp_diag[kp] ? 1.0 : 2.0) *
+                   (bp == kp ? 1.0 : 2.0) * 0.5;
+  const int A = args.sp_atom[2 * bp], B = args.sp_atom[2 * bp + 1];
+  const int C = args.sp_atom[2 * kp], D = args.sp_atom[2 * kp + 1];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double gA = f * acc[k], gB = f * acc[3 + k], gC = f * acc[6 + k];
+    atomicAdd(&args.grad[3 * A + k], gA);
+    atomicAdd(&args.grad[3 * B + k], gB);
+    atomicAdd(&args.grad[3 * C + k], gC);
+    atomicAdd(&args.grad[3 * D + k], -(gA + gB + gC));
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t grad2e_launch_thread(const Grad2eArgs& a, hipStream_t s) {
+  if (a.nquartets <= 0) return hipSuccess;
+  const dim3 grid = grid_1d((a.nquartets + 127) / 128, 128);
+  hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD>), grid, dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int LA, int LB, int LC, int LD>
 hipError_t grad2e_launch(const Grad2eArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   if (a.nquartets > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((grad2e_coop_kernel<LA, LB, LC, LD>), dim3((unsigned)a.nquartets), dim3(64), 0,
+  hipLaunchKernelGGL((grad2e_coop_kernel<LA, LB, LC, LD>), grid_1d(a.nquartets, 64), dim3(64), 0,
                      s, a);
   return hipGetLastError();
 }

@@ -49,7 +49,7 @@ __device__ inline int64_t pidx(int64_t i, int64_t j) {
 
 __global__ void sm_assemble_kernel(int kind, const double* __restrict__ J, double* __restrict__ out,
                                    int64_t nbf, int64_t nelem) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t e = flat_block() * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
   const int64_t npair = nbf * (nbf + 1) / 2;
   const int64_t t = e >> (2 * SM_LOGB);
@@ -256,8 +256,9 @@ __global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_tiles_kernel(const vo
 
 // Per-tile max |element| (one wave per tile).
 __global__ __launch_bounds__(64) void tile_maxabs_kernel(const double* __restrict__ M,
-                                                         double* __restrict__ tmax) {
-  const int64_t tile = blockIdx.x;
+                                                         double* __restrict__ tmax, int64_t ntiles) {
+  const int64_t tile = flat_block();
+  if (tile >= ntiles) return;
   const double* tp = M + (tile << (2 * SM_LOGB));
   double m = 0.0;
   for (int i = threadIdx.x; i < SM_B * SM_B; i += 64) m = fmax(m, fabs(tp[i]));
@@ -269,8 +270,9 @@ __global__ __launch_bounds__(64) void tile_maxabs_kernel(const double* __restric
 __global__ __launch_bounds__(256) void tile_pack_kernel(const double* __restrict__ M,
                                                         const uint8_t* __restrict__ fmt,
                                                         const int64_t* __restrict__ off,
-                                                        char* __restrict__ pool) {
-  const int64_t tile = blockIdx.x;
+                                                        char* __restrict__ pool, int64_t ntiles) {
+  const int64_t tile = flat_block();
+  if (tile >= ntiles) return;
   const uint8_t f = fmt[tile];
   if (f == TILE_ZERO) return;
   const double* tp = M + (tile << (2 * SM_LOGB));
@@ -372,7 +374,7 @@ int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* 
   if (Jsm == out) return set_error("qc_sm_assemble: in-place assembly is not supported");
   const int64_t T = sm_tiles_per_dim(nbf);
   const int64_t nelem = (T * (T + 1) / 2) << (2 * SM_LOGB);
-  hipLaunchKernelGGL(sm_assemble_kernel, dim3((unsigned)((nelem + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(sm_assemble_kernel, grid_1d((nelem + 255) / 256, 256), dim3(256), 0,
                      (hipStream_t)stream, kind, Jsm, out, nbf, nelem);
   return check(hipGetLastError(), "qc_sm_assemble launch");
 }
@@ -454,16 +456,18 @@ int qc_symv_mixed_tiles(const void* pool, const uint8_t* fmt, const int64_t* off
 
 int qc_sm_tile_maxabs(const double* M, int64_t nbf, double* tmax, void* stream) {
   const int64_t T = sm_tiles_per_dim(nbf);
-  hipLaunchKernelGGL(tile_maxabs_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(64), 0,
-                     (hipStream_t)stream, M, tmax);
+  const int64_t nt = T * (T + 1) / 2;
+  hipLaunchKernelGGL(tile_maxabs_kernel, grid_1d(nt, 64), dim3(64), 0, (hipStream_t)stream, M,
+                     tmax, nt);
   return check(hipGetLastError(), "qc_sm_tile_maxabs");
 }
 
 int qc_sm_pack(const double* M, int64_t nbf, const uint8_t* fmt, const int64_t* off, void* pool,
                void* stream) {
   const int64_t T = sm_tiles_per_dim(nbf);
-  hipLaunchKernelGGL(tile_pack_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0,
-                     (hipStream_t)stream, M, fmt, off, (char*)pool);
+  const int64_t nt = T * (T + 1) / 2;
+  hipLaunchKernelGGL(tile_pack_kernel, grid_1d(nt, 256), dim3(256), 0, (hipStream_t)stream, M,
+                     fmt, off, (char*)pool, nt);
   return check(hipGetLastError(), "qc_sm_pack");
 }
 

@@ -82,6 +82,19 @@ __device__ __forceinline__ void boys_fn(Real T, const Real* __restrict__ tab, Re
 // ---------------------------------------------------------------------------
 // Component tables
 // ---------------------------------------------------------------------------
+// Launch shape for nblocks work-groups of `block` lanes.  A dispatch packet
+// holds the grid size of each dimension as a 32-bit work-item count, so a
+// launch whose x extent would pass 2^31 work-items spills into y; kernels
+// take the flat work-group index from flat_block() and bound it themselves.
+inline dim3 grid_1d(int64_t nblocks, int block) {
+  const int64_t maxx = (int64_t(1) << 31) / block;
+  if (nblocks <= maxx) return dim3((unsigned)(nblocks > 0 ? nblocks : 1));
+  return dim3((unsigned)maxx, (unsigned)((nblocks + maxx - 1) / maxx));
+}
+__device__ __forceinline__ int64_t flat_block() {
+  return (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+}
+
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 __host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
 __host__ __device__ constexpr int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }

@@ -22,6 +22,25 @@ from jax_qc import _native as nat
 from jax_qc.integrals import engine as eng
 
 
+# Kernel form per (bra class, ket class) of the derivative-ERI contraction:
+# 1 one thread per quartet, 2 one wave per quartet, 0 the library's default.
+# Classes absent from the table use the default; a probe may set
+# GRAD_VARIANT_OVERRIDE to force one form for every class that has it.
+GRAD_VARIANTS: dict = {}
+GRAD_VARIANT_OVERRIDE = None
+
+
+def grad_variant_for(cab: int, ccd: int) -> int:
+    if GRAD_VARIANT_OVERRIDE is not None:
+        return GRAD_VARIANT_OVERRIDE if (GRAD_VARIANT_OVERRIDE != 1 or
+                                         (cab, ccd) in THREAD_CAPABLE) else 2
+    return GRAD_VARIANTS.get((cab, ccd), 0)
+
+
+# classes with a one-thread-per-quartet kernel (as built by csrc/build.py)
+THREAD_CAPABLE = {(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1), (4, 0)}
+
+
 def energy_weighted_density(C, orbital_energies, n_occ: int, factor: float = 2.0):
     """W = factor * sum_{i < n_occ} eps_i C_i C_i^T (rhf_grad.py:92-103)."""
     Co = C[:, :n_occ]
@@ -103,6 +122,7 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
             a.nk = len(Dkc)
             a.cx = cx
             a.grad = grad.data_ptr()
+            a.variant = grad_variant_for(L.cab, L.ccd)
             nat.call("qc_grad2e", L.cab, L.ccd, ctypes.byref(a), s)
     return grad.reshape(nat_, 3)
 

@@ -75,12 +75,17 @@ class ERIStore:
         self._work, self._x, self._y = {}, {}, {}
 
     def use_compact(self, tau_f32: float = 1e-4, tau_zero: float = 1e-14):
-        """Replace P (and J/K if present) by their compacted mixed forms."""
-        self.ensure_P()
-        self.P = self.compact(self.P, tau_f32, tau_zero) if isinstance(self.P, torch.Tensor) else self.P
-        if self.K is not None and isinstance(self.K, torch.Tensor):
+        """Replace the assembled supermatrices (P for RHF; J and K for UHF)
+        by their compacted mixed forms.  With neither P nor K assembled yet,
+        P is assembled first (the RHF case); a store that already holds K is
+        taken as UHF and no P is built."""
+        if self.P is None and self.K is None:
+            self.ensure_P()
+        if isinstance(self.P, torch.Tensor):
+            self.P = self.compact(self.P, tau_f32, tau_zero)
+        if isinstance(self.K, torch.Tensor):
             self.K = self.compact(self.K, tau_f32, tau_zero)
-        if self.J is not None and isinstance(self.J, torch.Tensor) and self.K is not None:
+        if isinstance(self.J, torch.Tensor) and self.K is not None:
             self.J = self.compact(self.J, tau_f32, tau_zero)
         return self
 
