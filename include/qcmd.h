@@ -57,6 +57,12 @@ typedef struct {
   const double* boys64;   /* filled in by the library */
   const float* boys32;
   int variant;            /* 0 built-in choice, 1 one thread per quartet, 2 one wave per quartet */
+  const int64_t* sp_base; /* mode 1, optional: per shell pair, element offset of the pair's
+                             supermatrix in out (many molecules in one launch: every
+                             quartet's bra and ket pair belong to one molecule, whose AO
+                             indices in sp_ao are molecule-local) */
+  int block;              /* one-thread-per-quartet launches: quartets per work-group,
+                             64 or 128 (0 = 128) */
 } qc_eri_args;
 
 /* One angular-momentum class of shell-quartet ERIs.  cab, ccd are pair-class
@@ -114,6 +120,44 @@ int qc_pack_density(const double* D, int64_t nbf, int nrhs, double* x, void* str
 int qc_unpack_fock(const double* H, const double* y, double scale, int64_t nbf, int nrhs,
                    double* F, void* stream);
 
+/* ---- many molecules per launch (batched SCF) -------------------------- */
+/* Each molecule m owns a tile-major supermatrix of molT[m] tile rows; the
+ * molecules' tiles are numbered consecutively (first global tile tile0[m]),
+ * as are their packed-vector rows (first row row0[m], 64 entries per row).
+ * A Fock build for all of them is pack -> tiles -> reduce -> unpack, four
+ * launches that a caller captures into one graph per SCF iteration.  The
+ * batch counterpart of jax_qc/scf/fock.py:41-47 applied to every molecule
+ * of a list (the reference runs molecules one at a time). */
+
+/* Per-tile max |element| and format-directed packing over ntiles
+ * consecutive 64 x 64 FP64 tiles (any number of molecules). */
+int qc_tiles_maxabs(const double* M, int64_t ntiles, double* tmax, void* stream);
+int qc_tiles_pack(const double* M, int64_t ntiles, const uint8_t* fmt, const int64_t* off,
+                  void* pool, void* stream);
+
+/* Host: item0[m] = first work item of molecule m for segment length seg
+ * (item0[nmol] = total, returned).  Host arrays. */
+int64_t qc_batch_items(const int64_t* molT, int nmol, int seg, int64_t* item0);
+
+/* Tile pass for all molecules in one persistent launch.  M: FP64 tiles
+ * (fmt = NULL) or a compacted pool with per-tile fmt/off (global tile
+ * numbering).  work: (nitems + ntiles) * nrhs * 64 doubles. */
+int qc_batch_symv_tiles(const void* M, const uint8_t* fmt, const int64_t* off, int nmol,
+                        const int64_t* molT, const int64_t* tile0, const int64_t* item0,
+                        const int64_t* row0, int64_t nitems, int64_t ntiles, int64_t npad,
+                        int nrhs, int seg, const double* x, void* work, void* stream);
+/* y[h][row*64 + r] for all nrows = row0[nmol] packed rows. */
+int qc_batch_symv_reduce(int nmol, const int64_t* molT, const int64_t* tile0, const int64_t* item0,
+                         const int64_t* row0, int64_t nrows, int64_t nitems, int64_t npad, int nrhs,
+                         int seg, double* y, void* work, void* stream);
+/* D: [nrhs][nmol][nmax][nmax], molecule m in its leading nbf[m] block. */
+int qc_batch_pack_density(const double* D, int64_t nmax, int nmol, const int64_t* nbf,
+                          const int64_t* row0, int nrhs, double* x, int64_t npad, void* stream);
+/* F[h][m] = H[m] + scale * unpack(y) inside the nbf[m] block, H[m] outside. */
+int qc_batch_unpack_fock(const double* H, const double* y, double scale, int64_t nmax, int nmol,
+                         const int64_t* nbf, const int64_t* row0, int nrhs, int64_t npad, double* F,
+                         void* stream);
+
 /* J_mn = sum_ls D_ls (mn|ls), K_mn = sum_ls D_ls (ms|ln) from a dense
  * tensor.  Replaces jax_qc/scf/fock.py:31-38 (_coulomb, _exchange). */
 int qc_dense_jk(const double* eri, const double* D, int64_t nbf, double* J, double* K,
@@ -146,7 +190,9 @@ typedef struct {
   double cx;
   double* grad;        /* natom x 3, accumulated (atomic adds) */
   int variant;         /* 0 library default, 1 one thread per quartet (low
-                          classes only), 2 one wave per quartet */
+                          classes only), 2 one wave per quartet with the bra
+                          side contracted with G first, 3 one wave per
+                          quartet, direct contraction */
 } qc_grad2e_args;
 
 /* 1/2 sum_quartets f sum_abcd G_abcd d(ab|cd)/dR for one class pair,

@@ -75,6 +75,9 @@ def _generate():
                 f"hipError_t grad2e_launch_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 f"  return grad2e_launch<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
                 "}\n"
+                f"hipError_t grad2e_launchw_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
+                "}\n"
                 f"hipError_t grad2e_launcht_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
                    if tag in GRAD_PER_THREAD else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")

@@ -175,6 +175,9 @@ int qc_eri(int cab, int ccd, int fp32, const qc_eri_args* args_in, void* stream)
   if (a.nquartets < 0 || (a.nquartets > 0 && a.nbra <= 0))
     return set_error("qc_eri: nquartets %lld with nbra %d", (long long)a.nquartets, a.nbra);
   if (a.variant < 0 || a.variant > 2) return set_error("qc_eri: variant %d", a.variant);
+  if (a.block != 0 && a.block != 64 && a.block != 128)
+    return set_error("qc_eri: block %d (need 0, 64 or 128)", a.block);
+  if (a.sp_base && a.mode != ERI_SUPER) return set_error("qc_eri: sp_base needs mode 1");
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
 #define QC_CALL(A, B, C, D)                                                                    \

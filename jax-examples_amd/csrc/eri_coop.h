@@ -320,7 +320,7 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
       out[((Lx * n + K) * n + I) * n + J] = val;
       out[((Lx * n + K) * n + J) * n + I] = val;
     } else {
-      store_super(args.out, I, J, K, Lx, val);
+      store_super(args.out + (args.sp_base ? args.sp_base[bp] : 0), I, J, K, Lx, val);
     }
   }
 }

@@ -263,7 +263,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
           o[((Lx * n + K) * n + I) * n + J] = val;
           o[((Lx * n + K) * n + J) * n + I] = val;
         } else {
-          store_super(args.out, I, J, K, Lx, val);
+          store_super(args.out + (args.sp_base ? args.sp_base[bp] : 0), I, J, K, Lx, val);
         }
       }
 }
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
 template <int LA, int LB, int LC, int LD, typename Real>
 hipError_t eri_launch_class(const EriArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
-  const int block = 128;
+  const int block = a.block == 64 ? 64 : 128;
   const dim3 grid = grid_1d((a.nquartets + block - 1) / block, block);
   hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, Real>), grid, dim3(block), 0, s, a);
   return hipGetLastError();

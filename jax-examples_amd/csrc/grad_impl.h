@@ -583,4 +583,319 @@ hipError_t grad2e_launch(const Grad2eArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// One wave per contracted quartet with the bra side contracted with G first.
+// Per bra primitive pair (alpha, p) and ket Cartesian pair cd:
+//   Y0_cd[tuv]    = sum_ab G_abcd E^{ab}_{tuv}                  (t+u+v <= la+lb)
+//   YA_cd,x[tuv]  = sum_ab G_abcd dE^{ab}/dA_x _{tuv}           (t+u+v <= la+lb+1)
+// with dE^{ab}/dA_x = 2 alpha E^{a+1,b} - a_x E^{a-1,b}.  Per primitive
+// quartet, with h_cd[tuv] = sum_{t'u'v'} (-1)^{t'+u'+v'} E^{cd}_{t'u'v'} R_{t+t',u+u',v+v'}
+// and hC_cd,x the same with the ket's C-derivative coefficients:
+//   dA_x  = pref sum_cd sum_tuv YA_cd,x[tuv] h_cd[tuv]
+//   dP_x  = pref sum_cd sum_tuv Y0_cd[tuv] h_cd[t+1,u,v]     (d/dA + d/dB: the
+//           bra distribution moves with P and d/dP_x R_{tuv} = R_{t+1,u,v})
+//   dC_x  = pref sum_cd sum_tuv Y0_cd[tuv] hC_cd,x[tuv]
+// then dB = dP - dA and dD = -(dP + dC) (translational invariance).  Y is
+// built once per bra primitive and reused by every ket primitive; the work
+// items (tuv, cd) are spread over all 64 lanes and ket pairs are taken in
+// chunks sized so that the LDS stays under GW_BUDGET doubles.
+// ---------------------------------------------------------------------------
+template <int L>
+struct HermDecode {
+  int tuv[nherm(L)];
+  constexpr HermDecode() : tuv() {
+    for (int s = 0; s <= L; ++s)
+      for (int t = s; t >= 0; --t)
+        for (int u = s - t; u >= 0; --u) tuv[herm_index(t, u, s - t - u)] = t | (u << 8) | ((s - t - u) << 16);
+  }
+};
+template <int L>
+__constant__ HermDecode<L> kHermDecode = HermDecode<L>();
+
+constexpr int GW_BUDGET = 6144;  // doubles of LDS per wave (48 KB)
+
+// Largest number of Hermite terms of one ket Cartesian pair (with one extra
+// order in some direction for the C-derivative lists).
+__host__ __device__ constexpr int tmax_terms(int lc, int ld) {
+  int m = 1;
+  for (int c = 0; c < ncart(lc); ++c)
+    for (int d = 0; d < ncart(ld); ++d) {
+      const int sx = cart_lx(lc, c) + cart_lx(ld, d), sy = cart_ly(lc, c) + cart_ly(ld, d);
+      const int sz = lc + ld - sx - sy;
+      const int n0 = (sx + 2) * (sy + 1) * (sz + 1), n1 = (sx + 1) * (sy + 2) * (sz + 1),
+                n2 = (sx + 1) * (sy + 1) * (sz + 2);
+      m = n0 > m ? n0 : m;
+      m = n1 > m ? n1 : m;
+      m = n2 > m ? n2 : m;
+    }
+  return m;
+}
+
+template <int LA, int LB, int LC, int LD>
+struct GradWave {
+  static constexpr int LAB = LA + LB, LCD = LC + LD, L1 = LAB + LCD + 1;
+  static constexpr int NB = ncart(LB), ND = ncart(LD);
+  static constexpr int NAB = ncart(LA) * NB, NCD = ncart(LC) * ND;
+  static constexpr int NHB0 = nherm(LAB), NHB1 = nherm(LAB + 1), NHR = nherm(L1);
+  static constexpr int EBS = (LA + 2) * (LB + 1) * (LAB + 2);  // E^{ij}_t: i <= la+1, j <= lb
+  static constexpr int EKS = (LC + 2) * (LD + 1) * (LCD + 2);
+  static constexpr int RB = L1 + 1;  // R^0 kept as an RB^3 box: index shifts are additive
+  // most ket terms (t', u', v') of one Cartesian pair cd: list 0 for h, list
+  // 1 + x for hC_x (one more term along x)
+  static constexpr int TMAX = tmax_terms(LC, LD);
+  static constexpr int TPER = 4 * TMAX * 3 / 2 + 4;  // doubles per cd: coefs, deltas (int), counts
+  static constexpr int FIXED = NAB * NCD + 3 * EBS + 3 * EKS + 2 * NHR + RB * RB * RB + (L1 + 1) + 8;
+  static constexpr int YPER = NHB0 + 3 * NHB1 + TPER;
+  static constexpr int CH0 = (GW_BUDGET - FIXED) / YPER;
+  static constexpr int NCDC = CH0 < 1 ? 1 : (CH0 > NCD ? NCD : CH0);
+  struct Shared {
+    double gam[NAB * NCD];
+    double eb[3][EBS];
+    double ek[3][EKS];
+    double r[2][NHR];
+    double rc[RB * RB * RB];
+    double f[L1 + 1];
+    double misc[8];  // X, Y, Z, pref
+    double y0[NCDC][NHB0];
+    double ya[3][NCDC][NHB1];
+    double tc[4][NCDC][TMAX];  // term coefficients (sign, pref and derivative factors folded in)
+    int td[4][NCDC][TMAX];     // term offsets in the R box
+    int tn[4][NCDC];           // term counts
+  };
+};
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
+  using W = GradWave<LA, LB, LC, LD>;
+  constexpr int LAB = W::LAB, LCD = W::LCD, L1 = W::L1, NB = W::NB, ND = W::ND;
+  constexpr int NAB = W::NAB, NCD = W::NCD, NHB0 = W::NHB0, NHB1 = W::NHB1, NHR = W::NHR;
+  constexpr int NCDC = W::NCDC;
+  __shared__ typename W::Shared sh;
+  const HermTable<L1>& HT = kHermTable<L1>;
+  const HermDecode<LAB + 1>& HD = kHermDecode<LAB + 1>;
+
+  const int lane = threadIdx.x;
+  const int64_t gid = flat_block();
+  if (gid >= args.nquartets) return;  // whole work-group (one wave)
+  int lo = 0, hi = args.nbra - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (args.offsets[mid] <= gid)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int bp = args.bra_list[lo];
+  const int kp = args.ket_list[args.kstart[lo] + (int)(gid - args.offsets[lo])];
+  const double* __restrict__ pp = args.pairs64;
+  const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
+  const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
+  const int64_t oa = args.sp_aoc[2 * bp], ob = args.sp_aoc[2 * bp + 1];
+  const int64_t oc = args.sp_aoc[2 * kp], od = args.sp_aoc[2 * kp + 1];
+  const int64_t nc = args.ncart;
+
+  for (int e = lane; e < NAB * NCD; e += 64) {
+    const int ab = e / NCD, cd = e % NCD;
+    const int64_t mu = oa + ab / NB, nu = ob + ab % NB, la = oc + cd / ND, si = od + cd % ND;
+    double k = args.dk0[mu * nc + la] * args.dk0[nu * nc + si] +
+               args.dk0[mu * nc + si] * args.dk0[nu * nc + la];
+    if (args.nk > 1)
+      k += args.dk1[mu * nc + la] * args.dk1[nu * nc + si] +
+           args.dk1[mu * nc + si] * args.dk1[nu * nc + la];
+    sh.gam[e] = args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k;
+  }
+
+  // E^{ij}_t of the bra / ket tables in LDS (zero for t > i + j).
+  auto eb = [&](int dim, int i, int j, int t) -> double {
+    return i < 0 ? 0.0 : sh.eb[dim][(i * (LB + 1) + j) * (LAB + 2) + t];
+  };
+  auto ek = [&](int dim, int i, int j, int t) -> double {
+    return i < 0 ? 0.0 : sh.ek[dim][(i * (LD + 1) + j) * (LCD + 2) + t];
+  };
+
+  double acc[9];  // dA xyz, dP xyz, dC xyz
+#pragma unroll
+  for (int i = 0; i < 9; ++i) acc[i] = 0.0;
+
+  for (int ib = 0; ib < npb; ++ib) {
+    const double* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
+    __syncthreads();
+    if (lane < 3) {
+      double E[LA + 2][LB + 1][LAB + 2];
+      hermite_e<LA + 1, LB, double>(E, P[PP_PAX + lane], P[PP_PBX + lane], 0.5 / P[PP_P]);
+#pragma unroll
+      for (int i = 0; i <= LA + 1; ++i)
+#pragma unroll
+        for (int j = 0; j <= LB; ++j)
+#pragma unroll
+          for (int t = 0; t <= LAB + 1; ++t) sh.eb[lane][(i * (LB + 1) + j) * (LAB + 2) + t] = E[i][j][t];
+    }
+    const double al2 = 2.0 * P[PP_ALPHA];
+    for (int cd0 = 0; cd0 < NCD; cd0 += NCDC) {
+      const int ncdc = min(NCDC, NCD - cd0);
+      __syncthreads();
+      // Y for this bra primitive and ket chunk
+      for (int it = lane; it < ncdc * NHB1; it += 64) {
+        const int cl = it / NHB1, k = it - cl * NHB1, cd = cd0 + cl;
+        const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
+        double y0 = 0.0, yx = 0.0, yy = 0.0, yz = 0.0;
+        for (int ab = 0; ab < NAB; ++ab) {
+          const int a = ab / NB, b = ab - a * NB;
+          const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
+          const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
+          if (t > ax + bx + 1 || u > ay + by + 1 || v > az + bz + 1) continue;
+          const double g = sh.gam[ab * NCD + cd];
+          const double e0x = eb(0, ax, bx, t), e0y = eb(1, ay, by, u), e0z = eb(2, az, bz, v);
+          const double eax = al2 * eb(0, ax + 1, bx, t) - ax * eb(0, ax - 1, bx, t);
+          const double eay = al2 * eb(1, ay + 1, by, u) - ay * eb(1, ay - 1, by, u);
+          const double eaz = al2 * eb(2, az + 1, bz, v) - az * eb(2, az - 1, bz, v);
+          y0 += g * e0x * e0y * e0z;
+          yx += g * eax * e0y * e0z;
+          yy += g * e0x * eay * e0z;
+          yz += g * e0x * e0y * eaz;
+        }
+        if (k < NHB0) sh.y0[cl][k] = y0;
+        sh.ya[0][cl][k] = yx;
+        sh.ya[1][cl][k] = yy;
+        sh.ya[2][cl][k] = yz;
+      }
+      for (int ik = 0; ik < npk; ++ik) {
+        const double* Q = pp + (int64_t)(pk0 + ik) * PP_STRIDE;
+        __syncthreads();
+        if (lane < 3) {
+          double E[LC + 2][LD + 1][LCD + 2];
+          hermite_e<LC + 1, LD, double>(E, Q[PP_PAX + lane], Q[PP_PBX + lane], 0.5 / Q[PP_P]);
+#pragma unroll
+          for (int i = 0; i <= LC + 1; ++i)
+#pragma unroll
+            for (int j = 0; j <= LD; ++j)
+#pragma unroll
+              for (int t = 0; t <= LCD + 1; ++t) sh.ek[lane][(i * (LD + 1) + j) * (LCD + 2) + t] = E[i][j][t];
+        } else if (lane == 32) {
+          const double p = P[PP_P], q = Q[PP_P];
+          const double pqm = p * q, ps = p + q, alpha = pqm / ps;
+          const double X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
+                       Z = P[PP_PX + 2] - Q[PP_PX + 2];
+          double F[L1 + 1];
+          boys_fn<L1, double>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
+          double pw = 1.0;
+#pragma unroll
+          for (int n = 0; n <= L1; ++n) {
+            sh.f[n] = pw * F[n];
+            pw *= -2.0 * alpha;
+          }
+          sh.misc[0] = X;
+          sh.misc[1] = Y;
+          sh.misc[2] = Z;
+          sh.misc[3] = 34.986836655249725693 / (pqm * sqrt(ps)) * P[PP_K] * Q[PP_K];
+        }
+        __syncthreads();
+        // ket term lists for this primitive and chunk: list 0 for h, 1 + x
+        // for hC_x; pref, the sign (-1)^{t'+u'+v'} and the derivative
+        // factors are folded into the coefficients
+        for (int w = lane; w < 4 * ncdc; w += 64) {  // 4 * ncdc may pass 64
+          const int l = w / ncdc, cl = w - l * ncdc, cd = cd0 + cl;
+          const int c = cd / ND, d = cd - c * ND;
+          const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
+          const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
+          const double pref = sh.misc[3], ga2 = 2.0 * Q[PP_ALPHA];
+          const int dim = l - 1;
+          const int ex2 = cx + dx + (dim == 0), ey2 = cy + dy + (dim == 1), ez2 = cz + dz + (dim == 2);
+          int n = 0;
+          for (int tt = 0; tt <= ex2; ++tt) {
+            const double e1 = dim == 0 ? ga2 * ek(0, cx + 1, dx, tt) - cx * ek(0, cx - 1, dx, tt)
+                                       : ek(0, cx, dx, tt);
+            for (int uu = 0; uu <= ey2; ++uu) {
+              const double e2 = e1 * (dim == 1 ? ga2 * ek(1, cy + 1, dy, uu) - cy * ek(1, cy - 1, dy, uu)
+                                               : ek(1, cy, dy, uu));
+              for (int vv = 0; vv <= ez2; ++vv) {
+                const double e3 = e2 * (dim == 2 ? ga2 * ek(2, cz + 1, dz, vv) - cz * ek(2, cz - 1, dz, vv)
+                                                 : ek(2, cz, dz, vv));
+                sh.tc[l][cl][n] = ((tt + uu + vv) & 1) ? -pref * e3 : pref * e3;
+                sh.td[l][cl][n] = (tt * W::RB + uu) * W::RB + vv;
+                ++n;
+              }
+            }
+          }
+          sh.tn[l][cl] = n;
+        }
+        // R^0_tuv (t+u+v <= L+1), levels n = L+1 .. 0, ping-pong buffers;
+        // level 0 also goes into the RB^3 box
+#pragma unroll
+        for (int n = L1; n >= 0; --n) {
+          const int nh = nherm(L1 - n);
+          double* cur = sh.r[n & 1];
+          const double* prv = sh.r[(n + 1) & 1];
+          for (int k = lane; k < nh; k += 64) {
+            double v;
+            if (k == 0) {
+              v = sh.f[n];
+            } else {
+              const HermRec hr = HT.rec[k];
+              v = sh.misc[hr.dir] * prv[hr.src1];
+              if (hr.src2 >= 0) v += (double)hr.mult * prv[hr.src2];
+            }
+            cur[k] = v;
+            if (n == 0) {
+              const int tuv = kHermDecode<L1>.tuv[k];
+              sh.rc[((tuv & 255) * W::RB + ((tuv >> 8) & 255)) * W::RB + (tuv >> 16)] = v;
+            }
+          }
+          __syncthreads();
+        }
+        const double* Rc = sh.rc;
+        for (int it = lane; it < ncdc * NHB1; it += 64) {
+          const int cl = it / NHB1, k = it - cl * NHB1;
+          const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
+          const double* Rb = Rc + (t * W::RB + u) * W::RB + v;
+          const int n0 = min(sh.tn[0][cl], W::TMAX);
+          double h = 0.0;
+          for (int j = 0; j < n0; ++j) h += sh.tc[0][cl][j] * Rb[sh.td[0][cl][j]];
+          acc[0] += sh.ya[0][cl][k] * h;
+          acc[1] += sh.ya[1][cl][k] * h;
+          acc[2] += sh.ya[2][cl][k] * h;
+          if (t > 0) acc[3] += sh.y0[cl][herm_index(t - 1, u, v)] * h;
+          if (u > 0) acc[4] += sh.y0[cl][herm_index(t, u - 1, v)] * h;
+          if (v > 0) acc[5] += sh.y0[cl][herm_index(t, u, v - 1)] * h;
+          if (k < NHB0) {
+            const double y0 = sh.y0[cl][k];
+#pragma unroll
+            for (int dim = 0; dim < 3; ++dim) {
+              const int nn = min(sh.tn[1 + dim][cl], W::TMAX);
+              double hc = 0.0;
+              for (int j = 0; j < nn; ++j) hc += sh.tc[1 + dim][cl][j] * Rb[sh.td[1 + dim][cl][j]];
+              acc[6 + dim] += y0 * hc;
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    for (int w = 32; w > 0; w >>= 1) acc[i] += __shfl_xor(acc[i], w);
+  if (lane == 0) {
+    const double f = (args.sp_diag[bp] ? 1.0 : 2.0) * (args.sp_diag[kp] ? 1.0 : 2.0) *
+                     (bp == kp ? 1.0 : 2.0) * 0.5;
+    const int A = args.sp_atom[2 * bp], B = args.sp_atom[2 * bp + 1];
+    const int C = args.sp_atom[2 * kp], D = args.sp_atom[2 * kp + 1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double gA = f * acc[k], gP = f * acc[3 + k], gC = f * acc[6 + k];
+      atomicAdd(&args.grad[3 * A + k], gA);
+      atomicAdd(&args.grad[3 * B + k], gP - gA);
+      atomicAdd(&args.grad[3 * C + k], gC);
+      atomicAdd(&args.grad[3 * D + k], -(gP + gC));
+    }
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t grad2e_launch_wave(const Grad2eArgs& a, hipStream_t s) {
+  if (a.nquartets <= 0) return hipSuccess;
+  hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD>), grid_1d(a.nquartets, 64), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace qcmd

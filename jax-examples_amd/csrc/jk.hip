@@ -174,19 +174,22 @@ __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile
   if (!diag) store_cols<NRHS, T>(col, tile, cg, ro, colbuf);
 }
 
-// One wave per work item (tile row R, a segment of its tiles).  Row
-// partials accumulate in registers over the segment and are reduced once at
-// the end by recursive halving over the 16 lanes sharing ro.  MIXED: tiles
-// are read through a format/offset directory (zero tiles skipped, FP32 or
-// FP64, FP64 accumulation either way).
+// One wave per work item (tile row R, a segment of its tiles) of one
+// supermatrix whose tiles start at global tile tile0 and whose packed vector
+// rows start at global row row0 (0 and 0 for a single matrix; many
+// molecules share one launch otherwise).  Row partials accumulate in
+// registers over the segment and are reduced once at the end by recursive
+// halving over the 16 lanes sharing ro.  MIXED: tiles are read through a
+// format/offset directory (zero tiles skipped, FP32 or FP64, FP64
+// accumulation either way).
 template <int NRHS, bool MIXED, bool NT>
-__global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_tiles_kernel(const void* __restrict__ Mv, int64_t T,
-                                                        const uint8_t* __restrict__ fmt,
-                                                        const int64_t* __restrict__ off,
-                                                        const double* __restrict__ x, int64_t npad,
-                                                        double* __restrict__ rowbuf,
-                                                        double* __restrict__ colbuf, int seg) {
-  const int64_t item = blockIdx.x;
+__device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T,
+                                          const uint8_t* __restrict__ fmt,
+                                          const int64_t* __restrict__ off,
+                                          const double* __restrict__ x, int64_t npad,
+                                          double* __restrict__ rowbuf, double* __restrict__ colbuf,
+                                          int seg, int64_t item, int64_t gitem, int64_t tile0,
+                                          int64_t row0) {
   int64_t lo = 0, hi = T - 1;
   while (lo < hi) {
     int64_t mid = (lo + hi + 1) >> 1;
@@ -204,14 +207,14 @@ __global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_tiles_kernel(const vo
   for (int h = 0; h < NRHS; ++h)
 #pragma unroll
     for (int k = 0; k < SYMV_ROWS; ++k) {
-      xr[h][k] = x[h * npad + R * SM_B + 4 * k + ro];
+      xr[h][k] = x[h * npad + (row0 + R) * SM_B + 4 * k + ro];
       rowacc[h][k] = 0.0;
     }
 
   for (int64_t C = C0; C < C1; ++C) {
-    const int64_t tile = R * (R + 1) / 2 + C;
+    const int64_t tile = tile0 + R * (R + 1) / 2 + C;
     const bool diag = C == R;
-    const double* xcb = x + C * SM_B;
+    const double* xcb = x + (row0 + C) * SM_B;
     if (!MIXED) {
       const double* tp = reinterpret_cast<const double*>(Mv) + (tile << (2 * SM_LOGB));
       symv_tile<NRHS, NT, double>(tp, tile, diag, xr, rowacc, xcb, npad, cg, ro, colbuf);
@@ -250,7 +253,51 @@ __global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_tiles_kernel(const vo
         rowacc[h][i] = keep + __shfl_xor(send, m);
       }
     }
-    rowbuf[(item * NRHS + h) * SM_B + 4 * cg + ro] = rowacc[h][0];
+    rowbuf[(gitem * NRHS + h) * SM_B + 4 * cg + ro] = rowacc[h][0];
+  }
+}
+
+template <int NRHS, bool MIXED, bool NT>
+__global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_tiles_kernel(const void* __restrict__ Mv, int64_t T,
+                                                        const uint8_t* __restrict__ fmt,
+                                                        const int64_t* __restrict__ off,
+                                                        const double* __restrict__ x, int64_t npad,
+                                                        double* __restrict__ rowbuf,
+                                                        double* __restrict__ colbuf, int seg) {
+  const int64_t item = blockIdx.x;
+  symv_item<NRHS, MIXED, NT>(Mv, T, fmt, off, x, npad, rowbuf, colbuf, seg, item, item, 0, 0);
+}
+
+// Last index i in [0, n) with pre[i] <= v (pre ascending, pre[0] = 0).
+__device__ __forceinline__ int find_seg(const int64_t* __restrict__ pre, int n, int64_t v) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Many supermatrices (one per molecule) in one persistent launch: waves
+// stride over the concatenated work items of all molecules.  Per molecule m:
+// T[m] tile rows, first global tile tile0[m], first global item item0[m]
+// (item0[nmol] = total), first global packed row row0[m].
+template <int NRHS, bool MIXED, bool NT>
+__global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_batch_kernel(
+    const void* __restrict__ Mv, const uint8_t* __restrict__ fmt, const int64_t* __restrict__ off,
+    const double* __restrict__ x, int64_t npad, double* __restrict__ rowbuf,
+    double* __restrict__ colbuf, int seg, int nmol, const int64_t* __restrict__ molT,
+    const int64_t* __restrict__ tile0, const int64_t* __restrict__ item0,
+    const int64_t* __restrict__ row0) {
+  const int64_t total = item0[nmol];
+  const int64_t stride = (int64_t)gridDim.x * gridDim.y;
+  for (int64_t g = flat_block(); g < total; g += stride) {
+    const int m = find_seg(item0, nmol, g);
+    symv_item<NRHS, MIXED, NT>(Mv, molT[m], fmt, off, x, npad, rowbuf, colbuf, seg, g - item0[m],
+                               g, tile0[m], row0[m]);
   }
 }
 
@@ -304,6 +351,66 @@ __global__ __launch_bounds__(256) void symv_reduce_kernel(int nrhs, int64_t T, i
   part[w][r] = acc;
   __syncthreads();
   if (w == 0) y[h * npad + R * SM_B + r] = part[0][r] + part[1][r] + part[2][r] + part[3][r];
+}
+
+// Batched reduce: one block per global packed row g (all molecules).
+__global__ __launch_bounds__(256) void symv_batch_reduce_kernel(
+    int nrhs, int64_t npad, const double* __restrict__ rowbuf, const double* __restrict__ colbuf,
+    double* __restrict__ y, int seg, int nmol, const int64_t* __restrict__ molT,
+    const int64_t* __restrict__ tile0, const int64_t* __restrict__ item0,
+    const int64_t* __restrict__ row0) {
+  const int64_t g = blockIdx.x;
+  const int h = blockIdx.y;
+  const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = find_seg(row0, nmol, g);
+  const int64_t R = g - row0[m], T = molT[m];
+  double acc = 0.0;
+  const int64_t i0 = item0[m] + item_off(R, seg), i1 = item0[m] + item_off(R + 1, seg);
+  for (int64_t it = i0 + w; it < i1; it += 4) acc += rowbuf[(it * nrhs + h) * SM_B + r];
+  for (int64_t Rp = R + 1 + w; Rp < T; Rp += 4) {
+    const int64_t tile = tile0[m] + Rp * (Rp + 1) / 2 + R;
+    acc += colbuf[(tile * nrhs + h) * SM_B + r];
+  }
+  __shared__ double part[4][64];
+  part[w][r] = acc;
+  __syncthreads();
+  if (w == 0) y[h * npad + g * SM_B + r] = part[0][r] + part[1][r] + part[2][r] + part[3][r];
+}
+
+// D: [nrhs][nmol][nmax][nmax] (molecule m uses its leading nbf[m] block) ->
+// packed x: [nrhs][npad], molecule m at rows row0[m] .. row0[m+1].
+__global__ void batch_pack_density_kernel(const double* __restrict__ D, int64_t nmax, int nmol,
+                                          const int64_t* __restrict__ nbf,
+                                          const int64_t* __restrict__ row0,
+                                          double* __restrict__ x, int64_t npad) {
+  const int64_t pg = flat_block() * blockDim.x + threadIdx.x;
+  const int h = blockIdx.z;
+  if (pg >= npad) return;
+  const int m = find_seg(row0, nmol, pg / SM_B);
+  const int64_t p = pg - row0[m] * SM_B, n = nbf[m], npair = n * (n + 1) / 2;
+  double v = 0.0;
+  if (p < npair) {
+    const int64_t k = tri_root(p), l = p - k * (k + 1) / 2;
+    v = D[(((int64_t)h * nmol + m) * nmax + k) * nmax + l] * (k == l ? 1.0 : 2.0);
+  }
+  x[h * npad + pg] = v;
+}
+
+// F[h][m][i][j] = H[m][i][j] + scale * y[h][row0[m]*64 + pair(i, j)] inside
+// molecule m's nbf[m] block, H[m][i][j] outside it.
+__global__ void batch_unpack_fock_kernel(const double* __restrict__ H, const double* __restrict__ y,
+                                         double scale, int64_t nmax, int nmol,
+                                         const int64_t* __restrict__ nbf,
+                                         const int64_t* __restrict__ row0, int64_t npad,
+                                         double* __restrict__ F) {
+  const int64_t e = flat_block() * blockDim.x + threadIdx.x;
+  const int h = blockIdx.z;
+  const int64_t per = nmax * nmax;
+  if (e >= nmol * per) return;
+  const int64_t m = e / per, ij = e - m * per, i = ij / nmax, j = ij - i * nmax;
+  double v = H ? H[e] : 0.0;
+  if (i < nbf[m] && j < nbf[m]) v += scale * y[h * npad + row0[m] * SM_B + pidx(i, j)];
+  F[(int64_t)h * nmol * per + e] = v;
 }
 
 __global__ void pack_density_kernel(const double* __restrict__ D, int64_t nbf, int nrhs,
@@ -452,6 +559,114 @@ int qc_symv_mixed_tiles(const void* pool, const uint8_t* fmt, const int64_t* off
                         int nrhs, const double* x, void* work, void* stream) {
   if (!fmt || !off) return set_error("qc_symv_mixed_tiles: null tile directory");
   return symv_phase(1, (const double*)pool, nbf, nrhs, x, nullptr, work, stream, fmt, off);
+}
+
+int qc_tiles_maxabs(const double* M, int64_t ntiles, double* tmax, void* stream) {
+  if (ntiles <= 0) return 0;
+  if (!M || !tmax) return set_error("qc_tiles_maxabs: null pointer");
+  hipLaunchKernelGGL(tile_maxabs_kernel, grid_1d(ntiles, 64), dim3(64), 0, (hipStream_t)stream, M,
+                     tmax, ntiles);
+  return check(hipGetLastError(), "qc_tiles_maxabs");
+}
+
+int qc_tiles_pack(const double* M, int64_t ntiles, const uint8_t* fmt, const int64_t* off,
+                  void* pool, void* stream) {
+  if (ntiles <= 0) return 0;
+  if (!M || !fmt || !off || !pool) return set_error("qc_tiles_pack: null pointer");
+  hipLaunchKernelGGL(tile_pack_kernel, grid_1d(ntiles, 256), dim3(256), 0, (hipStream_t)stream, M,
+                     fmt, off, (char*)pool, ntiles);
+  return check(hipGetLastError(), "qc_tiles_pack");
+}
+
+int64_t qc_batch_items(const int64_t* molT, int nmol, int seg, int64_t* item0) {
+  if (!molT || !item0 || nmol < 0 || seg < 1) return -1;
+  int64_t acc = 0;
+  for (int m = 0; m < nmol; ++m) {
+    item0[m] = acc;
+    acc += item_off(molT[m], seg);
+  }
+  item0[nmol] = acc;
+  return acc;
+}
+
+int qc_batch_symv_tiles(const void* M, const uint8_t* fmt, const int64_t* off, int nmol,
+                        const int64_t* molT, const int64_t* tile0, const int64_t* item0,
+                        const int64_t* row0, int64_t nitems, int64_t ntiles, int64_t npad,
+                        int nrhs, int seg, const double* x, void* work, void* stream) {
+  (void)ntiles;
+  if (nrhs < 1 || nrhs > 4) return set_error("qc_batch_symv_tiles: nrhs %d outside [1, 4]", nrhs);
+  if (seg < 1 || seg > 4096) return set_error("qc_batch_symv_tiles: seg %d", seg);
+  if (!M || !molT || !tile0 || !item0 || !row0 || !x || !work || (fmt && !off))
+    return set_error("qc_batch_symv_tiles: null pointer");
+  if (nmol <= 0 || nitems <= 0) return 0;
+  double* rowbuf = (double*)work;
+  double* colbuf = rowbuf + nitems * nrhs * SM_B;
+  // persistent: enough waves to fill the chip several times over, each
+  // striding over the items of every molecule
+  const int64_t waves = nitems < 16384 ? nitems : 16384;
+  hipStream_t s = (hipStream_t)stream;
+  switch (nrhs) {
+#define QC_L(N, MX, NT)                                                                           \
+  hipLaunchKernelGGL((symv_batch_kernel<N, MX, NT>), grid_1d(waves, 64), dim3(64), 0, s, M, fmt, \
+                     off, x, npad, rowbuf, colbuf, seg, nmol, molT, tile0, item0, row0)
+#define QC_S(N)              \
+  case N:                    \
+    if (fmt) {               \
+      if (g_symv_nt)         \
+        QC_L(N, true, true);  \
+      else                   \
+        QC_L(N, true, false); \
+    } else {                 \
+      if (g_symv_nt)         \
+        QC_L(N, false, true); \
+      else                   \
+        QC_L(N, false, false); \
+    }                        \
+    break;
+    QC_S(1) QC_S(2) QC_S(3) QC_S(4)
+#undef QC_S
+#undef QC_L
+  }
+  return check(hipGetLastError(), "qc_batch_symv_tiles");
+}
+
+int qc_batch_symv_reduce(int nmol, const int64_t* molT, const int64_t* tile0, const int64_t* item0,
+                         const int64_t* row0, int64_t nrows, int64_t nitems, int64_t npad, int nrhs,
+                         int seg, double* y, void* work, void* stream) {
+  if (nrhs < 1 || nrhs > 4) return set_error("qc_batch_symv_reduce: nrhs %d outside [1, 4]", nrhs);
+  if (!molT || !tile0 || !item0 || !row0 || !y || !work)
+    return set_error("qc_batch_symv_reduce: null pointer");
+  if (nmol <= 0 || nrows <= 0) return 0;
+  if (nrows * SM_B > npad) return set_error("qc_batch_symv_reduce: %lld rows exceed npad", (long long)nrows);
+  const double* rowbuf = (const double*)work;
+  const double* colbuf = rowbuf + nitems * nrhs * SM_B;
+  hipLaunchKernelGGL(symv_batch_reduce_kernel, dim3((unsigned)nrows, nrhs), dim3(256), 0,
+                     (hipStream_t)stream, nrhs, npad, rowbuf, colbuf, y, seg, nmol, molT, tile0,
+                     item0, row0);
+  return check(hipGetLastError(), "qc_batch_symv_reduce");
+}
+
+int qc_batch_pack_density(const double* D, int64_t nmax, int nmol, const int64_t* nbf,
+                          const int64_t* row0, int nrhs, double* x, int64_t npad, void* stream) {
+  if (!D || !nbf || !row0 || !x) return set_error("qc_batch_pack_density: null pointer");
+  if (nrhs < 1 || nrhs > 4 || nmol <= 0) return set_error("qc_batch_pack_density: nrhs %d nmol %d", nrhs, nmol);
+  dim3 g = grid_1d((npad + 255) / 256, 256);
+  g.z = nrhs;
+  hipLaunchKernelGGL(batch_pack_density_kernel, g, dim3(256), 0, (hipStream_t)stream, D, nmax, nmol,
+                     nbf, row0, x, npad);
+  return check(hipGetLastError(), "qc_batch_pack_density");
+}
+
+int qc_batch_unpack_fock(const double* H, const double* y, double scale, int64_t nmax, int nmol,
+                         const int64_t* nbf, const int64_t* row0, int nrhs, int64_t npad, double* F,
+                         void* stream) {
+  if (!y || !nbf || !row0 || !F) return set_error("qc_batch_unpack_fock: null pointer");
+  if (nrhs < 1 || nrhs > 4 || nmol <= 0) return set_error("qc_batch_unpack_fock: nrhs %d nmol %d", nrhs, nmol);
+  dim3 g = grid_1d((nmol * nmax * nmax + 255) / 256, 256);
+  g.z = nrhs;
+  hipLaunchKernelGGL(batch_unpack_fock_kernel, g, dim3(256), 0, (hipStream_t)stream, H, y, scale,
+                     nmax, nmol, nbf, row0, npad, F);
+  return check(hipGetLastError(), "qc_batch_unpack_fock");
 }
 
 int qc_sm_tile_maxabs(const double* M, int64_t nbf, double* tmax, void* stream) {

@@ -32,7 +32,7 @@ class QcEriArgs(ctypes.Structure):
         ("ket_list", c_void_p), ("offsets", c_void_p), ("kstart", c_void_p),
         ("nbra", c_int), ("nquartets", c_i64), ("mode", c_int), ("nbf", c_i64),
         ("out", c_void_p), ("schwarz", c_void_p), ("boys64", c_void_p), ("boys32", c_void_p),
-        ("variant", c_int),
+        ("variant", c_int), ("sp_base", c_void_p), ("block", c_int),
     ]
 
 
@@ -72,6 +72,17 @@ _SIGS = {
     "qc_grad1e": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                   c_int, c_int, c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p,
                   c_void_p],
+    "qc_tiles_maxabs": [c_void_p, c_i64, c_void_p, c_void_p],
+    "qc_tiles_pack": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p, c_void_p],
+    "qc_batch_symv_tiles": [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_i64, c_i64, c_i64, c_int, c_int, c_void_p, c_void_p,
+                            c_void_p],
+    "qc_batch_symv_reduce": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_i64, c_i64, c_i64,
+                             c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "qc_batch_pack_density": [c_void_p, c_i64, c_int, c_void_p, c_void_p, c_int, c_void_p, c_i64,
+                              c_void_p],
+    "qc_batch_unpack_fock": [c_void_p, c_void_p, c_double, c_i64, c_int, c_void_p, c_void_p,
+                             c_int, c_i64, c_void_p, c_void_p],
 }
 _OPTIONAL_SIGS = {}
 
@@ -98,6 +109,9 @@ def lib():
             L.qc_last_error.argtypes = []
             L.qc_symv_workspace_bytes.restype = c_i64
             L.qc_symv_workspace_bytes.argtypes = [c_i64, c_int]
+            L.qc_batch_items.restype = c_i64
+            L.qc_batch_items.argtypes = [ctypes.POINTER(c_i64), c_int, c_int,
+                                         ctypes.POINTER(c_i64)]
             for name, (argt, rt) in _OPTIONAL_SIGS.items():
                 fn = getattr(L, name)
                 fn.argtypes = argt
@@ -107,7 +121,8 @@ def lib():
 
 
 def exported_symbols():
-    return list(_SIGS) + ["qc_last_error", "qc_symv_workspace_bytes"] + list(_OPTIONAL_SIGS)
+    return (list(_SIGS) + ["qc_last_error", "qc_symv_workspace_bytes", "qc_batch_items"]
+            + list(_OPTIONAL_SIGS))
 
 
 def device() -> torch.device:

@@ -23,7 +23,9 @@ from jax_qc.integrals import engine as eng
 
 
 # Kernel form per (bra class, ket class) of the derivative-ERI contraction:
-# 1 one thread per quartet, 2 one wave per quartet, 0 the library's default.
+# 1 one thread per quartet, 2 one wave per quartet (bra side contracted with
+# G first), 3 one wave per quartet (direct contraction), 0 the library's
+# default.
 # Classes absent from the table use the default; a probe may set
 # GRAD_VARIANT_OVERRIDE to force one form for every class that has it.
 GRAD_VARIANTS: dict = {}

@@ -51,6 +51,35 @@ def pair_class(la: int, lb: int) -> int:
 CLASS_L = {0: (0, 0), 1: (1, 0), 2: (1, 1), 3: (2, 0), 4: (2, 1), 5: (2, 2)}
 
 
+def shell_pair_records(sa, a: int, b: int, prim_cut: float = 0.0) -> np.ndarray:
+    """Primitive-pair records of shells (a, b) (l_a >= l_b) from
+    ``shell_arrays`` output: [p, P(3), P-A(3), P-B(3), K, alpha] per row, K
+    holding both contraction coefficients and exp(-alpha beta/p |A-B|^2)
+    (gaussian_product.py:23-45, obara_saika.py:56-65)."""
+    ex, co, ce, off, npr = (sa[k] for k in ("exps", "coefs", "centers", "prim_off", "nprim"))
+    A, B = ce[a], ce[b]
+    ea = ex[off[a]:off[a] + npr[a]]
+    eb = ex[off[b]:off[b] + npr[b]]
+    ca = co[off[a]:off[a] + npr[a]]
+    cb = co[off[b]:off[b] + npr[b]]
+    al = np.repeat(ea, len(eb))
+    be = np.tile(eb, len(ea))
+    p = al + be
+    P = (al[:, None] * A + be[:, None] * B) / p[:, None]
+    AB2 = float(np.dot(A - B, A - B))
+    K = np.repeat(ca, len(cb)) * np.tile(cb, len(ca)) * np.exp(-al * be / p * AB2)
+    rec = np.empty((len(p), 12))
+    rec[:, 0] = p
+    rec[:, 1:4] = P
+    rec[:, 4:7] = P - A
+    rec[:, 7:10] = P - B
+    rec[:, 10] = K
+    rec[:, 11] = al
+    if prim_cut > 0.0:
+        rec = rec[np.abs(K) >= prim_cut]
+    return rec
+
+
 class PairTable:
     """Shell-pair and primitive-pair tables for one basis on one device."""
 
@@ -77,26 +106,7 @@ class PairTable:
         for i in range(self.nshell):
             for j in range(i + 1):
                 a, b = (i, j) if L[i] >= L[j] else (j, i)
-                A, B = ce[a], ce[b]
-                ea = ex[off[a]:off[a] + npr[a]]
-                eb = ex[off[b]:off[b] + npr[b]]
-                ca = co[off[a]:off[a] + npr[a]]
-                cb = co[off[b]:off[b] + npr[b]]
-                al = np.repeat(ea, len(eb))
-                be = np.tile(eb, len(ea))
-                p = al + be
-                P = (al[:, None] * A + be[:, None] * B) / p[:, None]
-                AB2 = float(np.dot(A - B, A - B))
-                K = np.repeat(ca, len(cb)) * np.tile(cb, len(ca)) * np.exp(-al * be / p * AB2)
-                rec = np.empty((len(p), 12))
-                rec[:, 0] = p
-                rec[:, 1:4] = P
-                rec[:, 4:7] = P - A
-                rec[:, 7:10] = P - B
-                rec[:, 10] = K
-                rec[:, 11] = al
-                if prim_cut > 0.0:
-                    rec = rec[np.abs(K) >= prim_cut]
+                rec = shell_pair_records(sa, a, b, prim_cut)
                 pairs.append(rec)
                 sp_off.append(npp)
                 sp_n.append(len(rec))
