@@ -193,6 +193,8 @@ typedef struct {
                           classes only), 2 one wave per quartet with the bra
                           side contracted with G first, 3 one wave per
                           quartet, direct contraction */
+  const double* schwarz; /* optional (with tau_density > 0): per shell pair Q */
+  double tau_density;    /* skip a quartet when max|G_abcd| Q_ab Q_cd < tau_density */
 } qc_grad2e_args;
 
 /* 1/2 sum_quartets f sum_abcd G_abcd d(ab|cd)/dR for one class pair,

@@ -240,6 +240,9 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   a.nk = in->nk;
   a.cx = in->cx;
   a.grad = in->grad;
+  a.schwarz = in->schwarz;
+  a.tau_density = in->tau_density;
+  if (a.tau_density > 0.0 && !a.schwarz) return set_error("qc_grad2e: tau_density without schwarz");
   a.boys64 = boys_table64();
   if (!a.boys64) return set_error("qc_grad2e: qc_init not called on the current device");
   if (in->variant < 0 || in->variant > 3) return set_error("qc_grad2e: variant %d", in->variant);

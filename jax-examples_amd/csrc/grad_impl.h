@@ -43,6 +43,8 @@ struct Grad2eArgs {
   double cx;
   double* grad;
   const double* boys64;
+  const double* schwarz;  // per shell pair Q (density screening), or null
+  double tau_density;
 };
 
 constexpr int GMAXP = 9;
@@ -411,6 +413,14 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
              args.dk1[mu * nc + si] * args.dk1[nu * nc + la];
       gam[ab][cd] = args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k;
     }
+  if (args.tau_density > 0.0) {
+    double gmax = 0.0;
+#pragma unroll
+    for (int ab = 0; ab < NAB; ++ab)
+#pragma unroll
+      for (int cd = 0; cd < NCD; ++cd) gmax = fmax(gmax, fabs(gam[ab][cd]));
+    if (gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return;
+  }
 
   double acc[9];
 #pragma unroll
@@ -704,6 +714,13 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
       k += args.dk1[mu * nc + la] * args.dk1[nu * nc + si] +
            args.dk1[mu * nc + si] * args.dk1[nu * nc + la];
     sh.gam[e] = args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k;
+  }
+  if (args.tau_density > 0.0) {
+    // each lane's maximum over the G entries it wrote, then over the wave
+    double gmax = 0.0;
+    for (int e = lane; e < NAB * NCD; e += 64) gmax = fmax(gmax, fabs(sh.gam[e]));
+    for (int w = 32; w > 0; w >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, w));
+    if (gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return;  // wave-uniform
   }
 
   // E^{ij}_t of the bra / ket tables in LDS (zero for t > i + j).

@@ -45,7 +45,8 @@ class QcGrad2eArgs(ctypes.Structure):
         ("bra_list", c_void_p), ("ket_list", c_void_p), ("offsets", c_void_p),
         ("kstart", c_void_p), ("nbra", c_int), ("nquartets", c_i64), ("ncart", c_i64),
         ("dj", c_void_p), ("dk0", c_void_p), ("dk1", c_void_p), ("nk", c_int),
-        ("cx", c_double), ("grad", c_void_p), ("variant", c_int),
+        ("cx", c_double), ("grad", c_void_p), ("variant", c_int), ("schwarz", c_void_p),
+        ("tau_density", c_double),
     ]
 
 

@@ -73,8 +73,10 @@ def _densities(mol, res):
 
 
 def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_table=None,
-                        parts=("1e", "2e")) -> torch.Tensor:
-    """sum D d(T+V) - sum W dS + 1/2 sum G dERI on the device, (natom, 3)."""
+                        parts=("1e", "2e"), tau_density: float = 0.0) -> torch.Tensor:
+    """sum D d(T+V) - sum W dS + 1/2 sum G dERI on the device, (natom, 3).
+    tau: Schwarz threshold of the quartet list; tau_density > 0 also skips
+    quartets with max|G_abcd| Q_ab Q_cd below it (density-weighted)."""
     dev = nat.device()
     nat.ensure_init(dev)
     pt = pair_table or eng.PairTable(basis, dev)
@@ -103,6 +105,7 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
                      pt.ncart, nat.ptr(Djc), nat.ptr(Wc), nat.ptr(grad), s)
     if "2e" in parts:
         plan = eng.plan_quartets(pt, tau)
+        Qd = torch.from_numpy(pt.Q).to(dev) if tau_density > 0.0 else None
         for L in plan.launches:
             a = nat.QcGrad2eArgs()
             a.pairs64 = pt.pairs64.data_ptr()
@@ -125,12 +128,17 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
             a.cx = cx
             a.grad = grad.data_ptr()
             a.variant = grad_variant_for(L.cab, L.ccd)
+            if Qd is not None:
+                a.schwarz = Qd.data_ptr()
+                a.tau_density = tau_density
             nat.call("qc_grad2e", L.cab, L.ccd, ctypes.byref(a), s)
     return grad.reshape(nat_, 3)
 
 
-def scf_gradient(mol, basis, res, tau: float = 1e-13, pair_table=None) -> torch.Tensor:
+def scf_gradient(mol, basis, res, tau: float = 1e-13, pair_table=None,
+                 tau_density: float = 0.0) -> torch.Tensor:
     """Full nuclear gradient (Eh/Bohr) of a converged RHF or UHF result."""
     Dj, Dk, cx, W = _densities(mol, res)
-    g = electronic_gradient(mol, basis, Dj, Dk, cx, W, tau, pair_table)
+    g = electronic_gradient(mol, basis, Dj, Dk, cx, W, tau, pair_table,
+                            tau_density=tau_density)
     return g + torch.from_numpy(nuclear_repulsion_gradient(mol)).to(g.device)
