@@ -145,6 +145,174 @@ def _path(mol, bs, mode, scf, tau_f32_tile):
     return ints, store, store.P, res, t_ints, t_scf, grad, t_grad
 
 
+def run_batch(args):
+    """256 random closed-shell molecules (2-12 atoms of H, C, N, O, seed 0),
+    even-tempered basis: one ERI launch per class pair for all molecules, one
+    graph-captured Fock build for all molecules per SCF iteration.  value =
+    molecule Fock builds/s (graph replays; 256 per replay)."""
+    import numpy as np
+    import torch
+
+    from jax_qc import autotune
+    from jax_qc import molecules as M
+    from jax_qc.batch import BatchedRHF
+
+    torch.cuda.set_device(0)
+    mols = M.random_molecules(256, seed=0)
+    out_modes = {}
+    tuning = None
+    for mode in ("fp64", "mixed"):
+        kw = dict(tau=1e-12) if mode == "fp64" else dict(tau=1e-12, tau32=1e-4,
+                                                          tile_f32=args.tile_f32)
+        if tuning is not None:
+            kw.update(eri_tuning=tuning["eri"], seg=tuning["seg"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = BatchedRHF(mols, **kw)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        if tuning is None and not args.no_tune:
+            tuning = autotune.autotune_batch(b)
+        D = torch.zeros(b.nmol, b.nmax, b.nmax, dtype=torch.float64, device="cuda")
+        b.fock(D)
+        for _ in range(args.warmup):
+            b.graph.replay()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            b.graph.replay()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        # the dominant kernel alone (eager launches bracketed by events)
+        ev = []
+        from jax_qc import _native as nat
+        for _ in range(10):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            nat.call("qc_batch_symv_tiles", nat.ptr(b.P), nat.ptr(b.fmt), nat.ptr(b.off), b.nmol,
+                     nat.ptr(b.molT), nat.ptr(b.tile0), nat.ptr(b.item0), nat.ptr(b.row0),
+                     b.nitems, b.ntiles, b.npad, 1, b.seg, nat.ptr(b.x), nat.ptr(b.work),
+                     nat.stream_ptr())
+            e1.record()
+            ev.append((e0, e1))
+        torch.cuda.synchronize()
+        k_ms = float(np.mean([a.elapsed_time(c) for a, c in ev]))
+        r = b.run(conv=1e-10)
+        out_modes[mode] = dict(rate=b.nmol * args.steps / dt, ms=dt / args.steps * 1e3,
+                               kernel_ms=k_ms, bytes=b.matrix_bytes, setup_s=t_setup,
+                               eri_s=b.eri_seconds, n64=b.plan.n64, n32=b.plan.n32,
+                               tiles=b.tile_counts, scf_s=r.scf_seconds, iters=r.iterations,
+                               converged=int(r.converged.sum()), E=r.energies,
+                               conv_mask=r.converged, nbf=(int(b.nbf_m.min()), int(b.nbf_m.max()),
+                                                           float(b.nbf_m.mean())))
+        del b
+        torch.cuda.empty_cache()
+    f, m = out_modes["fp64"], out_modes["mixed"]
+    both = f["conv_mask"] & m["conv_mask"]
+    dE = float(np.abs(f["E"][both] - m["E"][both]).max())
+    # the faster of the paths that hold |dE| <= 1 uEh against the FP64 path
+    pick = "mixed" if (dE <= 1e-6 and m["rate"] > f["rate"]) else "fp64"
+    P = out_modes[pick]
+    achieved = P["bytes"] / (P["kernel_ms"] * 1e-3) / 1e9
+    cfg = {"workload": "256 random closed-shell molecules (H,C,N,O; 2-12 atoms; seed 0), "
+                       "even-tempered basis, batched RHF Fock build (one graph per SCF iteration)",
+           "n_molecules": 256, "n_basis_min_max_mean": P["nbf"], "path": pick,
+           "max_dE_mixed_vs_fp64_over_converged": dE, "converged_both": int(both.sum()),
+           "tuning": None if tuning is None else {
+               "seg": tuning["seg"], "eri": {f"{a},{c}": list(v) for (a, c), v in tuning["eri"].items()}}}
+    for mode, r in out_modes.items():
+        for k, v in r.items():
+            if k in ("E", "conv_mask"):
+                continue
+            cfg[f"{mode}_{k}"] = round(v, 6) if isinstance(v, float) else v
+    out = {
+        "metric": "molecule Fock builds/s, batched RHF (256 molecules per graph launch)",
+        "value": round(P["rate"], 1), "unit": "molecule Fock builds/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(P["ms"], 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if pick == "fp64" else "f32/f64 split, f64 accumulate",
+        "data": "synthetic: seeded random molecules and seeded even-tempered basis",
+        "config": cfg,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "symv_batch_kernel", "bytes_per_launch": P["bytes"],
+                     "avg_launch_ms": round(P["kernel_ms"], 5)},
+    }
+    if not args.no_cpu:
+        import numpy as np  # noqa: F811
+
+        # the oracle's per-molecule Fock build at the batch's mean basis size
+        n = int(round(P["nbf"][2]))
+        cb = cpu_baseline(n, rows=min(24, n))
+        cb["sample"] = "one molecule of the mean basis size: " + cb["sample"]
+        cb["unit"] = "molecule Fock builds/s"
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+def run_water16_uhf(args):
+    """(H2O)_16 UHF triplet (grid 4x2x2, seed 2, even-tempered, 480
+    functions): energy + analytic gradient through the plain FP64 path
+    (Schwarz 1e-12, no other screening) and the mixed path (FP32-split ERIs
+    below 1e-4, FP32 supermatrix tiles, Schwarz 1e-11 and density-weighted
+    screening 1e-10 in the gradient), then BFGS steps on the mixed path with
+    the FP64 path evaluated at every accepted geometry."""
+    import numpy as np
+    import torch
+
+    from jax_qc import molecules as M
+    from jax_qc.core.types import CalcConfig
+    from jax_qc.geomopt.optimizer import energy_and_gradient, optimize_bfgs
+
+    torch.cuda.set_device(0)
+    mol = M.water_cluster((4, 2, 2), seed=2, spin=2)
+    cfgc = CalcConfig(method="uhf", basis="even-tempered", scf_conv=1e-10, max_scf_iter=200)
+    strict = dict(tau=1e-12, tau32=None, tile_f32=None, grad_tau=1e-12, grad_tau_density=0.0)
+    mixed = dict(tau=1e-12, tau32=1e-4, tile_f32=args.tile_f32, grad_tau=1e-11,
+                 grad_tau_density=1e-10)
+
+    def timed(coords, opts):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        E, g, _, res = energy_and_gradient(coords, mol, "even-tempered", cfgc, **opts)
+        torch.cuda.synchronize()
+        return E, g, time.perf_counter() - t, res.converged
+
+    steps = []
+
+    def cb(step, coords, E, g, res):
+        E64, g64, t64, c64 = timed(coords, strict)
+        row = dict(step=step, E_mixed=E, E_fp64=E64, dE=abs(E - E64),
+                   max_dgrad=float(np.abs(g - g64).max()), gmax=float(np.abs(g64).max()),
+                   fp64_s=round(t64, 3), fp64_converged=bool(c64))
+        steps.append(row)
+        print(json.dumps({"bfgs_step": row}), flush=True)
+
+    c0 = np.asarray(mol.coords, dtype=np.float64)
+    E64, g64, t64, _ = timed(c0, strict)
+    Em, gm, tm, _ = timed(c0, mixed)
+    print(json.dumps({"start": {"fp64_s": t64, "mixed_s": tm, "dE": abs(Em - E64),
+                                "max_dgrad": float(np.abs(gm - g64).max())}}), flush=True)
+    res = optimize_bfgs(mol, "even-tempered", "uhf", max_steps=args.bfgs_steps, grad_tol=1e-4,
+                        scf_conv=1e-10, verbose=0, callback=cb, **mixed)
+    out = {
+        "metric": "energy+gradient wall time, (H2O)_16 UHF triplet, mixed vs FP64 GPU",
+        "value": round(tm, 3), "unit": "s", "n_gpus": 1, "steps": len(steps), "warmup": 0,
+        "ms_per_step": round(tm * 1e3, 1), "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f32/f64 split, f64 accumulate",
+        "data": "synthetic: seeded (H2O)_16 geometry and seeded even-tempered basis",
+        "config": {"workload": "(H2O)_16 UHF triplet energy + analytic gradient, 480 functions",
+                   "fp64_energy_grad_s": round(t64, 3), "mixed_energy_grad_s": round(tm, 3),
+                   "speedup": round(t64 / tm, 3), "dE_start": abs(Em - E64),
+                   "max_dgrad_start": float(np.abs(gm - g64).max()),
+                   "bfgs_steps": steps, "bfgs_converged": bool(res.converged),
+                   "bfgs_final_energy": res.energy,
+                   "max_dE_over_steps": max([r["dE"] for r in steps] + [0.0]),
+                   "max_dgrad_over_steps": max([r["max_dgrad"] for r in steps] + [0.0])},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,11 +330,21 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a plain device copy of this many MiB first, as the byte "
                          "calibration kernel for HBM traffic counters")
+    ap.add_argument("--workload", choices=["water8", "batch", "water16-uhf"], default="water8",
+                    help="water8: (H2O)_8 RHF Fock builds/s (default); batch: 256 random "
+                         "molecules, batched Fock builds + SCF; water16-uhf: (H2O)_16 UHF "
+                         "triplet energy+gradient, mixed vs FP64, and BFGS steps")
+    ap.add_argument("--bfgs-steps", type=int, default=10)
+    ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the quartet "
                                    "list is design-only (see DESIGN.md)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "water16-uhf":
+        return run_water16_uhf(args)
 
     import numpy as np
     import torch
@@ -218,7 +396,9 @@ def main():
         dE = abs(results["mixed"]["energy"] - results["fp64"]["energy"])
         dG = float(np.abs(results["mixed"]["_grad"] - results["fp64"]["_grad"]).max())
     if args.mode == "auto":
-        pick = "mixed" if (dE is not None and dE <= 1e-6) else "fp64"
+        # the faster of the paths that hold |dE| <= 1 uEh against the FP64 path
+        pick = ("mixed" if (dE is not None and dE <= 1e-6
+                            and results["mixed"]["rate"] > results["fp64"]["rate"]) else "fp64")
     else:
         pick = args.mode
     P = results[pick]
@@ -244,8 +424,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if pick == "fp64" else "f32/f64 split (f32 below Schwarz bound 1e-4 and "
-                                               "in supermatrix tiles below 1e-4; f64 accumulate)",
+        "dtype": "f64" if pick == "fp64" else (
+            "f32/f64 split (f32 ERI sums below Schwarz bound 1e-4, f32 supermatrix tiles below "
+            f"max|element| {args.tile_f32:g}; f64 accumulate)"),
         "data": "synthetic: seeded (H2O)_8 geometry and seeded even-tempered basis",
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

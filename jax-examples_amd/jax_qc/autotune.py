@@ -71,6 +71,75 @@ def autotune_eri(pt, plan, out: torch.Tensor | None = None, mode: int = eng.MODE
     return timings
 
 
+# --- batched SCF: work-group shape per quartet class, Fock segment length ---
+
+# (variant, quartets per work-group): one thread per quartet in 64- or
+# 128-lane work-groups, or one wave per quartet
+BATCH_ERI_CHOICES = ((1, 64), (1, 128), (2, 0))
+BATCH_SEGS = (1, 2, 4, 8, 16)
+
+
+def autotune_batch(b, repeats: int = 2, eri_choices=BATCH_ERI_CHOICES, segs=BATCH_SEGS,
+                   verbose: bool = False) -> dict:
+    """Time every class launch of a ``BatchedRHF``'s quartet plan with each
+    (kernel, work-group) choice and the batched Fock build with each segment
+    length; returns {"eri": {(cab, ccd): (variant, block)}, "seg": s,
+    "timings": ...} for ``BatchedRHF(eri_tuning=..., seg=...)``.  A class
+    pair with FP64 and FP32 launches gets the choice with the smaller sum."""
+    J = torch.zeros(int(b.bpt.sm_base_m[-1]), dtype=torch.float64, device=b.dev)
+    per_class: Dict[Tuple[int, int], Dict[Tuple[int, int], float]] = {}
+    timings = {}
+    for L in b.plan.launches:
+        res = {}
+        for v, blk in eri_choices:
+            b.eri_launch(L, J, v, blk)  # warm (code object load)
+            torch.cuda.synchronize(b.dev)
+            t = time.perf_counter()
+            for _ in range(repeats):
+                b.eri_launch(L, J, v, blk)
+            torch.cuda.synchronize(b.dev)
+            res[(v, blk)] = (time.perf_counter() - t) / repeats * 1e3
+        timings[(L.cab, L.ccd, L.fp32)] = res
+        acc = per_class.setdefault((L.cab, L.ccd), {})
+        for k, ms in res.items():
+            acc[k] = acc.get(k, 0.0) + ms
+        if verbose:
+            print(f"autotune batch {(L.cab, L.ccd, L.fp32)} nq={L.nquartets}: "
+                  + ", ".join(f"v{v}/{blk}={ms:.3f} ms" for (v, blk), ms in res.items()), flush=True)
+    del J
+    eri = {k: min(v, key=v.get) for k, v in per_class.items()}
+    seg_ms = {}
+    D = torch.zeros(b.nmol, b.nmax, b.nmax, dtype=torch.float64, device=b.dev)
+    for sg in segs:
+        b.set_seg(sg)
+        b.fock(D)  # captures
+        torch.cuda.synchronize(b.dev)
+        t = time.perf_counter()
+        for _ in range(max(5, repeats)):
+            b.graph.replay()
+        torch.cuda.synchronize(b.dev)
+        seg_ms[sg] = (time.perf_counter() - t) / max(5, repeats) * 1e3
+    seg = min(seg_ms, key=seg_ms.get)
+    b.set_seg(seg)
+    if verbose:
+        print("autotune batch seg: " + ", ".join(f"{k}={v:.3f} ms" for k, v in seg_ms.items()),
+              flush=True)
+    return {"eri": eri, "seg": seg, "timings": {"eri": timings, "seg": seg_ms}}
+
+
+def save_batch(tuning: dict, path: str) -> None:
+    with open(path, "w") as fh:
+        json.dump({"eri": {f"{a},{c}": list(v) for (a, c), v in tuning["eri"].items()},
+                   "seg": tuning["seg"]}, fh, indent=1)
+
+
+def load_batch(path: str) -> dict:
+    with open(path) as fh:
+        d = json.load(fh)
+    return {"eri": {tuple(int(x) for x in k.split(",")): tuple(v) for k, v in d["eri"].items()},
+            "seg": int(d["seg"])}
+
+
 def save(path: str) -> None:
     with open(path, "w") as fh:
         json.dump({f"{a},{b}": v for (a, b), v in eng.ERI_VARIANT.items()}, fh, indent=1)

@@ -220,25 +220,16 @@ class BatchedRHF:
         self.tau, self.tau32 = tau, tau32
         self.plan = plan_quartets_batched(self.bpt, tau, tau32)
         J = torch.zeros(int(self.bpt.sm_base_m[-1]), dtype=torch.float64, device=self.dev)
-        tuning = eri_tuning or {}
-        s = nat.stream_ptr()
+        self.eri_tuning = dict(eri_tuning or {})
         for L in self.plan.launches:
-            a = self.bpt.base_args(eng.MODE_SUPER, out=J)
-            a.bra_list = L.bra.data_ptr()
-            a.ket_list = L.ket.data_ptr()
-            a.offsets = L.offsets.data_ptr()
-            a.kstart = L.kstart.data_ptr()
-            a.nbra = L.nbra
-            a.nquartets = L.nquartets
-            v, blk = tuning.get((L.cab, L.ccd), (eng.variant_for(L.cab, L.ccd), 0))
-            a.variant = v
-            a.block = blk
-            nat.call("qc_eri", L.cab, L.ccd, L.fp32, ctypes.byref(a), s)
+            v, blk = self.eri_tuning.get((L.cab, L.ccd), (eng.variant_for(L.cab, L.ccd), 0))
+            self.eri_launch(L, J, v, blk)
         torch.cuda.synchronize(self.dev)
         self.eri_seconds = time.perf_counter() - t0
         # ---- P = J - 1/4 K per molecule ------------------------------------
         P = torch.empty_like(J)
         base = self.bpt.sm_base_m
+        s = nat.stream_ptr()
         for i in range(n):
             nat.call("qc_sm_assemble", 0, J.data_ptr() + 8 * int(base[i]),
                      P.data_ptr() + 8 * int(base[i]), int(self.nbf_m[i]), s)
@@ -252,6 +243,19 @@ class BatchedRHF:
         self.assembly_seconds = time.perf_counter() - t0 - self.eri_seconds
         self._setup_fock(seg)
         self.graph = None
+
+    def eri_launch(self, L: eng.ClassLaunch, J: torch.Tensor, variant: int, block: int = 0):
+        """One class launch of the batch plan into the J pool ``J``."""
+        a = self.bpt.base_args(eng.MODE_SUPER, out=J)
+        a.bra_list = L.bra.data_ptr()
+        a.ket_list = L.ket.data_ptr()
+        a.offsets = L.offsets.data_ptr()
+        a.kstart = L.kstart.data_ptr()
+        a.nbra = L.nbra
+        a.nquartets = L.nquartets
+        a.variant = variant
+        a.block = block
+        nat.call("qc_eri", L.cab, L.ccd, L.fp32, ctypes.byref(a), nat.stream_ptr())
 
     # -- mixed-precision tiles ----------------------------------------------
     def _compact(self, tau_f32: float, tau_zero: float = 1e-14):

@@ -12,9 +12,11 @@ point (used to compare precision modes step by step).
 
 from __future__ import annotations
 
+import dataclasses
 from typing import Callable, NamedTuple, Optional
 
 import numpy as np
+import torch
 
 from jax_qc.basis import build_basis_set
 from jax_qc.core.types import CalcConfig
@@ -49,8 +51,12 @@ def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12,
     res = run_scf(m, ints, config)
     g = scf_gradient(m, bs, res, grad_tau, pair_table=getattr(ints.ERI, "pt", None),
                      tau_density=grad_tau_density)
+    # drop the integral store (tens of GB at 480 functions) before the
+    # caller's next evaluation
+    res = dataclasses.replace(res, ERI=None)
     out = float(res.energy), g.cpu().numpy(), m, res
     del ints
+    torch.cuda.empty_cache()
     return out
 
 
