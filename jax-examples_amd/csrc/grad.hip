@@ -41,6 +41,8 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
     }
   double gA[3] = {0, 0, 0};  // S and T parts (B = -A)
   double vA[3] = {0, 0, 0}, vB[3] = {0, 0, 0};
+  // the nuclei are split over blockIdx.y; the S and T terms belong to y == 0
+  const bool st = blockIdx.y == 0;
 
   for (int ip = 0; ip < np; ++ip) {
     const double* P = pp + (int64_t)(p0 + ip) * PP_STRIDE;
@@ -79,6 +81,7 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
     for (int a = 0; a < NA; ++a)
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
+        if (!st) continue;
         const int ai[3] = {CX[LA][a], CY[LA][a], CZ[LA][a]};
         const int bi[3] = {CX[LB][b], CY[LB][b], CZ[LB][b]};
 #pragma unroll
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
         }
       }
     // nuclear attraction: E up to (LA+1, LB+1) (already in Ex..), R up to L+1
-    for (int c = 0; c < natom; ++c) {
+    for (int c = blockIdx.y; c < natom; c += gridDim.y) {
       const double X = P[PP_PX] - axyz[3 * c], Y = P[PP_PX + 1] - axyz[3 * c + 1],
                    Z = P[PP_PX + 2] - axyz[3 * c + 2];
       double F[L + 2];
@@ -197,7 +200,9 @@ int qc_grad1e(int cab, const double* pairs64, const int* sp_prim_off, const int*
   if (!boys) return set_error("qc_grad1e: qc_init not called on the current device");
   if (!D || !W || !grad) return set_error("qc_grad1e: null matrix");
   hipStream_t s = (hipStream_t)stream;
-  dim3 grid((npairs + 127) / 128), block(128);
+  // pairs over x, nuclei over y (a pair's nuclear loop is the long serial
+  // part: few pairs of the high classes, many atoms)
+  dim3 grid((npairs + 127) / 128, natom < 64 ? (natom > 0 ? natom : 1) : 64), block(128);
 #define QC_G1(A, B)                                                                             \
   case (A) * ((A) + 1) / 2 + (B):                                                               \
     hipLaunchKernelGGL((grad1e_kernel<A, B>), grid, block, 0, s, pairs64, sp_prim_off, sp_nprim, \
@@ -247,10 +252,9 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   if (!a.boys64) return set_error("qc_grad2e: qc_init not called on the current device");
   if (in->variant < 0 || in->variant > 3) return set_error("qc_grad2e: variant %d", in->variant);
   // default (measured on (H2O)_8): one thread per quartet for ket (s s)
-  // classes up to (dp|ss) and for (ps|ps), one wave with the bra side
-  // contracted first for the rest
-  const int var = in->variant != 0 ? in->variant
-                                   : ((ccd == 0 && cab <= 4) || (cab == 1 && ccd == 1) ? 1 : 2);
+  // classes up to (dp|ss), one wave with the bra side contracted first for
+  // the rest
+  const int var = in->variant != 0 ? in->variant : (ccd == 0 && cab <= 4 ? 1 : 2);
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
 #define QC_GC(A, B, C, D)                                                  \

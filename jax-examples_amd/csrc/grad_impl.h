@@ -623,7 +623,8 @@ struct HermDecode {
 template <int L>
 __constant__ HermDecode<L> kHermDecode = HermDecode<L>();
 
-constexpr int GW_BUDGET = 6144;  // doubles of LDS per wave (48 KB)
+constexpr int GW_BUDGET = 6144;     // doubles of LDS per wave (48 KB)
+constexpr int GW_KC_BUDGET = 2560;  // batching ket primitives stops at 20 KB (8 waves per CU)
 
 // Largest number of Hermite terms of one ket Cartesian pair (with one extra
 // order in some direction for the C-derivative lists).
@@ -654,24 +655,44 @@ struct GradWave {
   // most ket terms (t', u', v') of one Cartesian pair cd: list 0 for h, list
   // 1 + x for hC_x (one more term along x)
   static constexpr int TMAX = tmax_terms(LC, LD);
-  static constexpr int TPER = 4 * TMAX * 3 / 2 + 4;  // doubles per cd: coefs, deltas (int), counts
-  static constexpr int FIXED = NAB * NCD + 3 * EBS + 3 * EKS + 2 * NHR + RB * RB * RB + (L1 + 1) + 8;
-  static constexpr int YPER = NHB0 + 3 * NHB1 + TPER;
-  static constexpr int CH0 = (GW_BUDGET - FIXED) / YPER;
-  static constexpr int NCDC = CH0 < 1 ? 1 : (CH0 > NCD ? NCD : CH0);
+  // LDS doubles: fixed part, per ket pair of a chunk, per ket primitive of
+  // a batch, per (ket pair, ket primitive)
+  static constexpr int FIX = NAB * NCD + 3 * EBS + 8;
+  static constexpr int YPER = NHB0 + 3 * NHB1;
+  static constexpr int KPER = 3 * EKS + 2 * NHR + RB * RB * RB + (L1 + 1) + 4;
+  static constexpr int TPER = 4 * TMAX * 3 / 2 + 4;
+  static constexpr int lds(int nc, int kc) { return FIX + nc * YPER + kc * (KPER + nc * TPER); }
+  // as many ket primitives per batch (at most GMAXP) as keep the whole ket
+  // side within GW_KC_BUDGET (occupancy); ket pairs are then chunked to fit
+  // GW_BUDGET.  A batch shares the barrier-separated phases (ket E, Boys,
+  // term lists, R levels) among its primitive quartets.
+  static constexpr int pick_kc() {
+    int kc = GMAXP;
+    while (kc > 1 && lds(NCD, kc) > GW_KC_BUDGET) --kc;
+    return kc;
+  }
+  // batches of 2-6 cost more in occupancy than they save in barriers
+  // (measured): below 7 the kernel takes one ket primitive at a time
+  static constexpr int KC = pick_kc() >= 7 ? pick_kc() : 1;
+  static constexpr int pick_nc() {
+    int nc = NCD;
+    while (nc > 1 && lds(nc, KC) > GW_BUDGET) --nc;
+    return nc;
+  }
+  static constexpr int NCDC = pick_nc();
   struct Shared {
     double gam[NAB * NCD];
     double eb[3][EBS];
-    double ek[3][EKS];
-    double r[2][NHR];
-    double rc[RB * RB * RB];
-    double f[L1 + 1];
-    double misc[8];  // X, Y, Z, pref
     double y0[NCDC][NHB0];
     double ya[3][NCDC][NHB1];
-    double tc[4][NCDC][TMAX];  // term coefficients (sign, pref and derivative factors folded in)
-    int td[4][NCDC][TMAX];     // term offsets in the R box
-    int tn[4][NCDC];           // term counts
+    double ek[KC][3][EKS];
+    double r[2][KC][NHR];
+    double rc[KC][RB * RB * RB];
+    double f[KC][L1 + 1];
+    double misc[KC][4];           // X, Y, Z, pref
+    double tc[KC][4][NCDC][TMAX];  // term coefficients (sign, pref and derivative factors folded in)
+    int td[KC][4][NCDC][TMAX];     // term offsets in the R box
+    int tn[KC][4][NCDC];           // term counts
   };
 };
 
@@ -680,7 +701,7 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
   using W = GradWave<LA, LB, LC, LD>;
   constexpr int LAB = W::LAB, LCD = W::LCD, L1 = W::L1, NB = W::NB, ND = W::ND;
   constexpr int NAB = W::NAB, NCD = W::NCD, NHB0 = W::NHB0, NHB1 = W::NHB1, NHR = W::NHR;
-  constexpr int NCDC = W::NCDC;
+  constexpr int NCDC = W::NCDC, KC = W::KC, RB = W::RB, TMAX = W::TMAX;
   __shared__ typename W::Shared sh;
   const HermTable<L1>& HT = kHermTable<L1>;
   const HermDecode<LAB + 1>& HD = kHermDecode<LAB + 1>;
@@ -727,8 +748,8 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
   auto eb = [&](int dim, int i, int j, int t) -> double {
     return i < 0 ? 0.0 : sh.eb[dim][(i * (LB + 1) + j) * (LAB + 2) + t];
   };
-  auto ek = [&](int dim, int i, int j, int t) -> double {
-    return i < 0 ? 0.0 : sh.ek[dim][(i * (LD + 1) + j) * (LCD + 2) + t];
+  auto ek = [&](int kk, int dim, int i, int j, int t) -> double {
+    return i < 0 ? 0.0 : sh.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 2) + t];
   };
 
   double acc[9];  // dA xyz, dP xyz, dC xyz
@@ -777,19 +798,25 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
         sh.ya[1][cl][k] = yy;
         sh.ya[2][cl][k] = yz;
       }
-      for (int ik = 0; ik < npk; ++ik) {
-        const double* Q = pp + (int64_t)(pk0 + ik) * PP_STRIDE;
+      for (int ik0 = 0; ik0 < npk; ik0 += KC) {
+        const int nk = min(KC, npk - ik0);
         __syncthreads();
-        if (lane < 3) {
+        // ket E tables (lanes 3 kk + dim) and Boys values (lanes 32 + kk)
+        if (lane < 3 * nk) {
+          const int kk = lane / 3, dim = lane - 3 * kk;
+          const double* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
           double E[LC + 2][LD + 1][LCD + 2];
-          hermite_e<LC + 1, LD, double>(E, Q[PP_PAX + lane], Q[PP_PBX + lane], 0.5 / Q[PP_P]);
+          hermite_e<LC + 1, LD, double>(E, Q[PP_PAX + dim], Q[PP_PBX + dim], 0.5 / Q[PP_P]);
 #pragma unroll
           for (int i = 0; i <= LC + 1; ++i)
 #pragma unroll
             for (int j = 0; j <= LD; ++j)
 #pragma unroll
-              for (int t = 0; t <= LCD + 1; ++t) sh.ek[lane][(i * (LD + 1) + j) * (LCD + 2) + t] = E[i][j][t];
-        } else if (lane == 32) {
+              for (int t = 0; t <= LCD + 1; ++t) sh.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 2) + t] = E[i][j][t];
+        }
+        if (lane >= 32 && lane - 32 < nk) {
+          const int kk = lane - 32;
+          const double* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
           const double p = P[PP_P], q = Q[PP_P];
           const double pqm = p * q, ps = p + q, alpha = pqm / ps;
           const double X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
@@ -799,76 +826,80 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
           double pw = 1.0;
 #pragma unroll
           for (int n = 0; n <= L1; ++n) {
-            sh.f[n] = pw * F[n];
+            sh.f[kk][n] = pw * F[n];
             pw *= -2.0 * alpha;
           }
-          sh.misc[0] = X;
-          sh.misc[1] = Y;
-          sh.misc[2] = Z;
-          sh.misc[3] = 34.986836655249725693 / (pqm * sqrt(ps)) * P[PP_K] * Q[PP_K];
+          sh.misc[kk][0] = X;
+          sh.misc[kk][1] = Y;
+          sh.misc[kk][2] = Z;
+          sh.misc[kk][3] = 34.986836655249725693 / (pqm * sqrt(ps)) * P[PP_K] * Q[PP_K];
         }
         __syncthreads();
-        // ket term lists for this primitive and chunk: list 0 for h, 1 + x
-        // for hC_x; pref, the sign (-1)^{t'+u'+v'} and the derivative
+        // ket term lists per (ket primitive, list, ket pair): list 0 for h,
+        // 1 + x for hC_x; pref, the sign (-1)^{t'+u'+v'} and the derivative
         // factors are folded into the coefficients
-        for (int w = lane; w < 4 * ncdc; w += 64) {  // 4 * ncdc may pass 64
-          const int l = w / ncdc, cl = w - l * ncdc, cd = cd0 + cl;
+        for (int w = lane; w < nk * 4 * ncdc; w += 64) {
+          const int kk = w / (4 * ncdc), r0 = w - kk * 4 * ncdc, l = r0 / ncdc, cl = r0 - l * ncdc;
+          const int cd = cd0 + cl;
           const int c = cd / ND, d = cd - c * ND;
           const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
           const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
-          const double pref = sh.misc[3], ga2 = 2.0 * Q[PP_ALPHA];
+          const double pref = sh.misc[kk][3];
+          const double ga2 = 2.0 * pp[(int64_t)(pk0 + ik0 + kk) * PP_STRIDE + PP_ALPHA];
           const int dim = l - 1;
           const int ex2 = cx + dx + (dim == 0), ey2 = cy + dy + (dim == 1), ez2 = cz + dz + (dim == 2);
           int n = 0;
           for (int tt = 0; tt <= ex2; ++tt) {
-            const double e1 = dim == 0 ? ga2 * ek(0, cx + 1, dx, tt) - cx * ek(0, cx - 1, dx, tt)
-                                       : ek(0, cx, dx, tt);
+            const double e1 = dim == 0 ? ga2 * ek(kk, 0, cx + 1, dx, tt) - cx * ek(kk, 0, cx - 1, dx, tt)
+                                       : ek(kk, 0, cx, dx, tt);
             for (int uu = 0; uu <= ey2; ++uu) {
-              const double e2 = e1 * (dim == 1 ? ga2 * ek(1, cy + 1, dy, uu) - cy * ek(1, cy - 1, dy, uu)
-                                               : ek(1, cy, dy, uu));
+              const double e2 = e1 * (dim == 1 ? ga2 * ek(kk, 1, cy + 1, dy, uu) - cy * ek(kk, 1, cy - 1, dy, uu)
+                                               : ek(kk, 1, cy, dy, uu));
               for (int vv = 0; vv <= ez2; ++vv) {
-                const double e3 = e2 * (dim == 2 ? ga2 * ek(2, cz + 1, dz, vv) - cz * ek(2, cz - 1, dz, vv)
-                                                 : ek(2, cz, dz, vv));
-                sh.tc[l][cl][n] = ((tt + uu + vv) & 1) ? -pref * e3 : pref * e3;
-                sh.td[l][cl][n] = (tt * W::RB + uu) * W::RB + vv;
+                const double e3 = e2 * (dim == 2 ? ga2 * ek(kk, 2, cz + 1, dz, vv) - cz * ek(kk, 2, cz - 1, dz, vv)
+                                                 : ek(kk, 2, cz, dz, vv));
+                sh.tc[kk][l][cl][n] = ((tt + uu + vv) & 1) ? -pref * e3 : pref * e3;
+                sh.td[kk][l][cl][n] = (tt * RB + uu) * RB + vv;
                 ++n;
               }
             }
           }
-          sh.tn[l][cl] = n;
+          sh.tn[kk][l][cl] = n;
         }
-        // R^0_tuv (t+u+v <= L+1), levels n = L+1 .. 0, ping-pong buffers;
-        // level 0 also goes into the RB^3 box
+        // R^0_tuv (t+u+v <= L+1) of every primitive quartet of the batch,
+        // levels n = L+1 .. 0, ping-pong buffers; level 0 also goes into the
+        // RB^3 boxes
 #pragma unroll
         for (int n = L1; n >= 0; --n) {
           const int nh = nherm(L1 - n);
-          double* cur = sh.r[n & 1];
-          const double* prv = sh.r[(n + 1) & 1];
-          for (int k = lane; k < nh; k += 64) {
+          for (int w = lane; w < nk * nh; w += 64) {
+            const int kk = w / nh, k = w - kk * nh;
+            double* cur = sh.r[n & 1][kk];
+            const double* prv = sh.r[(n + 1) & 1][kk];
             double v;
             if (k == 0) {
-              v = sh.f[n];
+              v = sh.f[kk][n];
             } else {
               const HermRec hr = HT.rec[k];
-              v = sh.misc[hr.dir] * prv[hr.src1];
+              v = sh.misc[kk][hr.dir] * prv[hr.src1];
               if (hr.src2 >= 0) v += (double)hr.mult * prv[hr.src2];
             }
             cur[k] = v;
             if (n == 0) {
               const int tuv = kHermDecode<L1>.tuv[k];
-              sh.rc[((tuv & 255) * W::RB + ((tuv >> 8) & 255)) * W::RB + (tuv >> 16)] = v;
+              sh.rc[kk][((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16)] = v;
             }
           }
           __syncthreads();
         }
-        const double* Rc = sh.rc;
-        for (int it = lane; it < ncdc * NHB1; it += 64) {
-          const int cl = it / NHB1, k = it - cl * NHB1;
+        for (int it = lane; it < nk * ncdc * NHB1; it += 64) {
+          const int kk = it / (ncdc * NHB1), r0 = it - kk * ncdc * NHB1;
+          const int cl = r0 / NHB1, k = r0 - cl * NHB1;
           const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
-          const double* Rb = Rc + (t * W::RB + u) * W::RB + v;
-          const int n0 = min(sh.tn[0][cl], W::TMAX);
+          const double* Rb = sh.rc[kk] + (t * RB + u) * RB + v;
+          const int n0 = min(sh.tn[kk][0][cl], TMAX);
           double h = 0.0;
-          for (int j = 0; j < n0; ++j) h += sh.tc[0][cl][j] * Rb[sh.td[0][cl][j]];
+          for (int j = 0; j < n0; ++j) h += sh.tc[kk][0][cl][j] * Rb[sh.td[kk][0][cl][j]];
           acc[0] += sh.ya[0][cl][k] * h;
           acc[1] += sh.ya[1][cl][k] * h;
           acc[2] += sh.ya[2][cl][k] * h;
@@ -879,9 +910,9 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
             const double y0 = sh.y0[cl][k];
 #pragma unroll
             for (int dim = 0; dim < 3; ++dim) {
-              const int nn = min(sh.tn[1 + dim][cl], W::TMAX);
+              const int nn = min(sh.tn[kk][1 + dim][cl], TMAX);
               double hc = 0.0;
-              for (int j = 0; j < nn; ++j) hc += sh.tc[1 + dim][cl][j] * Rb[sh.td[1 + dim][cl][j]];
+              for (int j = 0; j < nn; ++j) hc += sh.tc[kk][1 + dim][cl][j] * Rb[sh.td[kk][1 + dim][cl][j]];
               acc[6 + dim] += y0 * hc;
             }
           }

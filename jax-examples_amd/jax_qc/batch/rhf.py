@@ -388,6 +388,7 @@ class BatchedRHF:
         E = self._energy(D, F0)
         last = E.cpu().numpy()
         focks, errs = [], []
+        gram = np.zeros((self.nmol, 0, 0))  # <e_i, e_j> per molecule, in step with the ring
         done = np.zeros(self.nmol, dtype=bool)
         n_it = np.zeros(self.nmol, dtype=np.int64)
         E_final = last.copy()
@@ -401,11 +402,17 @@ class BatchedRHF:
             if len(focks) == diis_space:
                 focks.pop(0)
                 errs.pop(0)
+                gram = gram[:, 1:, 1:]
             focks.append(F)
             errs.append(err)
+            row = torch.stack([(x * err).sum(dim=(1, 2)) for x in errs], 1).cpu().numpy()
+            m = len(errs)
+            G = np.empty((self.nmol, m, m))
+            G[:, :-1, :-1] = gram
+            G[:, -1, :] = row
+            G[:, :, -1] = row
+            gram = G
             if len(focks) >= 2:
-                Eh = torch.stack(errs, 1).reshape(self.nmol, len(errs), -1)
-                G = (Eh @ Eh.transpose(1, 2)).cpu().numpy()
                 coef = np.zeros((self.nmol, len(focks)))
                 m = len(focks)
                 for i in range(self.nmol):

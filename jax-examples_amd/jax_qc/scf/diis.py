@@ -19,6 +19,10 @@ class DIISHistory:
     max_size: int
     focks: List[torch.Tensor] = field(default_factory=list)
     errors: List[torch.Tensor] = field(default_factory=list)
+    # Gram matrix <e_i, e_j> of the stored errors, kept in step with the
+    # ring: a push adds one row (a matrix-vector product) instead of a
+    # (k x n^2)(n^2 x k) product, which BLAS runs as a single tile.
+    gram: Optional[np.ndarray] = None
 
     @property
     def size(self) -> int:
@@ -40,14 +44,26 @@ def _push(h: DIISHistory, F, e) -> DIISHistory:
     if h.size == h.max_size:
         h.focks.pop(0)
         h.errors.pop(0)
+        if h.gram is not None:
+            h.gram = h.gram[1:, 1:]
     h.focks.append(F)
     h.errors.append(e)
+    v = e.reshape(-1)
+    row = torch.stack([torch.dot(x.reshape(-1), v) for x in h.errors]).cpu().numpy()
+    m = len(h.errors)
+    G = np.empty((m, m))
+    if m > 1:
+        G[:-1, :-1] = h.gram
+    G[-1, :] = row
+    G[:, -1] = row
+    h.gram = G
     return h
 
 
-def _solve_diis(errors) -> Optional[np.ndarray]:
-    E = torch.stack([e.reshape(-1) for e in errors])
-    G = (E @ E.T).cpu().numpy()
+def _solve_diis(errors, G=None) -> Optional[np.ndarray]:
+    if G is None:
+        E = torch.stack([e.reshape(-1) for e in errors])
+        G = (E @ E.T).cpu().numpy()
     m = G.shape[0]
     B = np.empty((m + 1, m + 1))
     B[:m, :m] = G
@@ -63,7 +79,7 @@ def _solve_diis(errors) -> Optional[np.ndarray]:
 
 
 def _extrapolate(h: DIISHistory):
-    c = _solve_diis(h.errors)
+    c = _solve_diis(h.errors, h.gram)
     if c is None:
         return None
     ct = torch.as_tensor(c, dtype=h.focks[0].dtype, device=h.focks[0].device)
