@@ -31,6 +31,15 @@ for _p in (REPO, os.path.join(REPO, "jax-examples_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+# HBM bytes per launch of the Fock-build tile kernel on the (H2O)_8 workload,
+# from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script, each
+# scaled by the same counters on a 1 GiB device copy (FETCH x2.00, WRITE
+# x1.00); profiles/symv_tiles_water8_*.md hold the commands and counts.
+MEASURED_TRAFFIC = {"fp64": (1639021.6 * 2.0 + 54269.0) * 1024,
+                    "mixed": (1152157.2 * 2.0 + 63845.3) * 1024}
 
 
 def cpu_baseline(n: int, rows: int = 24, seed: int = 0):
@@ -113,7 +122,56 @@ def _fock_rate(store, M, D, H, n, steps, warmup):
     return steps / dt, dt, avg_ms
 
 
-def _path(mol, bs, mode, scf, tau_f32_tile):
+def eri_report(store):
+    """Per-class ERI kernel time (HIP events around a re-run of each class
+    launch into a scratch supermatrix) and algorithmic FLOP rate, by
+    precision; printed as one JSON line."""
+    import numpy as np
+    import torch
+
+    from jax_qc.integrals import engine as eng
+
+    pt = store.pt
+    J = torch.zeros(eng.sm_numel(pt.nbf), dtype=torch.float64, device=pt.dev)
+    sp_n = pt.sp_nprim.cpu().numpy()
+    mean_np = {c: float(sp_n[pt.class_pairs[c]].mean()) if len(pt.class_pairs[c]) else 0.0
+               for c in range(6)}
+    rows, tot = [], {0: [0.0, 0.0], 1: [0.0, 0.0]}
+    for L in store.plan.launches:
+        sub = eng.QuartetPlan(launches=[L])
+        eng.run_eri(pt, sub, eng.MODE_SUPER, J)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            eng.run_eri(pt, sub, eng.MODE_SUPER, J)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        la, lb = eng.CLASS_L[L.cab]
+        lc, ld = eng.CLASS_L[L.ccd]
+        fl = (L.nquartets * mean_np[L.cab] * mean_np[L.ccd]
+              * eng.flops_per_primitive_quartet(la, lb, lc, ld))
+        tot[L.fp32][0] += fl
+        tot[L.fp32][1] += ms
+        rows.append({"class": [la, lb, lc, ld], "fp32": L.fp32, "quartets": L.nquartets,
+                     "ms": round(ms, 4), "tflops": round(fl / (ms * 1e-3) / 1e12, 3),
+                     "variant": eng.variant_for(L.cab, L.ccd)})
+    rows.sort(key=lambda r: -r["ms"])
+    summary = {}
+    for p, (fl, ms) in tot.items():
+        if ms:
+            peak = FP32_PEAK_TFLOPS if p else FP64_PEAK_TFLOPS
+            summary["fp32" if p else "fp64"] = {
+                "tflop": round(fl / 1e12, 4), "ms": round(ms, 3),
+                "tflops": round(fl / (ms * 1e-3) / 1e12, 3),
+                "frac_of_peak": round(fl / (ms * 1e-3) / 1e12 / peak, 4)}
+    print(json.dumps({"eri_report": summary, "classes": rows}), flush=True)
+    del J
+    torch.cuda.empty_cache()
+
+
+def _path(mol, bs, mode, scf, tau_f32_tile, report=False):
     """Integrals (+ SCF) for one precision mode; returns (ints, store, M, res)."""
     import torch
 
@@ -124,6 +182,8 @@ def _path(mol, bs, mode, scf, tau_f32_tile):
     t0 = time.perf_counter()
     ints = jax_qc.compute_integrals(mol, bs, eri="store", tau=1e-12, tau32=tau32)
     store = ints.ERI
+    if report:
+        eri_report(store)
     store.ensure_P()
     store.release_J()
     if mode == "mixed":
@@ -335,6 +395,9 @@ def main():
                          "molecules, batched Fock builds + SCF; water16-uhf: (H2O)_16 UHF "
                          "triplet energy+gradient, mixed vs FP64, and BFGS steps")
     ap.add_argument("--bfgs-steps", type=int, default=10)
+    ap.add_argument("--eri-report", action="store_true",
+                    help="also print per-class ERI kernel times and FLOP rates (one JSON line "
+                         "per precision mode)")
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -372,7 +435,7 @@ def main():
     results = {}
     for mode in modes:
         ints, store, Mx, res, t_ints, t_scf, grad, t_grad = _path(mol, bs, mode, scf,
-                                                                  args.tile_f32)
+                                                                  args.tile_f32, args.eri_report)
         D = (res.state.density.contiguous() if res is not None
              else torch.eye(n, dtype=torch.float64, device=dev) * 0.1)
         rate, dt, avg_ms = _fock_rate(store, Mx, D, ints.H_core.contiguous(), n, args.steps,
@@ -430,7 +493,11 @@ def main():
         "data": "synthetic: seeded (H2O)_8 geometry and seeded even-tempered basis",
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (round(MEASURED_TRAFFIC[pick])
+                                 if grid == (2, 2, 2) and (pick == "fp64" or args.tile_f32 == 1e-2)
+                                 else None),
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated",
                      "kernel": "symv_tiles_kernel<1,%s>" % ("true" if pick == "mixed" else "false"),
                      "bytes_per_launch": P["bytes"], "avg_launch_ms": round(P["avg_ms"], 5)},
     }
