@@ -1,21 +1,21 @@
-// eri_coop.h -- wave-cooperative McMurchie-Davidson ERIs for the high
-// angular-momentum classes (total L >= 5), where one thread per quartet
-// would need several hundred live FP64 values (R_tuv alone has 165 entries
-// at L = 8) and spills to scratch.
+// eri_coop.h -- wave-cooperative McMurchie-Davidson ERIs for the classes
+// where one thread per quartet would need several hundred live values (R_tuv
+// alone has 165 entries at L = 8) and spills to scratch.
 //
-// One 64-lane wave per contracted quartet.  Per ket primitive pair:
-//   stage 1: Boys values for every bra primitive pair of the current chunk,
-//            then the R^n_tuv recursion (jax_qc/integrals/mcmurchie_davidson.py:
-//            113-172) run level by level with the (bra prim, tuv) entries of a
-//            level spread over the lanes, levels ping-ponged through LDS;
-//   stage 2: lane (cd, g) owns ket Cartesian pair cd and bra primitive pairs
-//            g, g+G, ...; it forms the ket-contracted Hermite integrals
-//            h_tuv = sum_{t'u'v'} (-1)^{t'+u'+v'} E^{cd}_{t'u'v'} R_{t+t',u+u',v+v'}
-//            for t+u+v <= la+lb and accumulates E^{ab} . h for every bra
-//            Cartesian pair ab in registers (obara_saika.py:285-331).
-// After all primitives the G partial blocks of each cd are summed through
-// LDS, the Cartesian block is taken to the spherical basis by all lanes, and
-// written out like the per-thread kernels.
+// One 64-lane wave per contracted quartet; every lane owns a fixed set of
+// elements of the Cartesian block V[ab][cd] and accumulates them in
+// registers.  Per bra primitive pair: per bra Cartesian pair ab the list of
+// its nonzero E^{ab}_{tuv} (coefficient, Hermite index).  Per batch of ket
+// primitive pairs (as many as the LDS budget allows): Boys values, the
+// R^n_tuv recursion (jax_qc/integrals/mcmurchie_davidson.py:113-172) level
+// by level with the (primitive, tuv) entries of a level spread over the
+// lanes and R^0 kept as a box so that index shifts add; per ket Cartesian
+// pair cd the list of its terms pref (-1)^{t'+u'+v'} E^{cd}_{t'u'v'};
+//   h_cd[tuv] = sum_{t'u'v'} c_{t'u'v'} R_{t+t',u+u',v+v'}   (t+u+v <= la+lb)
+// over all lanes; then V[ab][cd] += sum_{tuv} E^{ab}_{tuv} h_cd[tuv]
+// (obara_saika.py:285-331).  After all primitives the block goes through
+// LDS to the spherical basis (all lanes) and is written out like the
+// per-thread kernels.
 #pragma once
 #include "eri_impl.h"
 
@@ -62,36 +62,107 @@ struct HermTable {
 template <int L>
 __constant__ HermTable<L> kHermTable = HermTable<L>();
 
-template <int LA, int LB, int LC, int LD, typename Real>
-struct CoopShared {
-  static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
-  static constexpr int ESA = (LA + 1) * (LB + 1) * (LAB + 1);
-  static constexpr int ESC = (LC + 1) * (LD + 1) * (LCD + 1);
-  static constexpr int NHL = nherm(L);
-  static constexpr int NAB = ncart(LA) * ncart(LB), NCD = ncart(LC) * ncart(LD);
-  static constexpr int RED = 64 * NAB > NAB * NCD ? 64 * NAB : NAB * NCD;
-  static constexpr int RB = 2 * COOP_MAXP * NHL > RED ? 2 * COOP_MAXP * NHL : RED;
-  Real eb[COOP_MAXP][3][ESA];
-  Real ek[3][ESC];
-  Real f[COOP_MAXP][L + 1];  // (-2 alpha)^n F_n
-  Real pq[COOP_MAXP][4];     // X, Y, Z, prefactor
-  Real r[RB];                // two R levels, later the reduction buffer
+// (t, u, v) of Hermite index k, packed t | u << 8 | v << 16.
+template <int L>
+struct HermDecode {
+  int tuv[nherm(L)];
+  constexpr HermDecode() : tuv() {
+    for (int s = 0; s <= L; ++s)
+      for (int t = s; t >= 0; --t)
+        for (int u = s - t; u >= 0; --u) tuv[herm_index(t, u, s - t - u)] = t | (u << 8) | ((s - t - u) << 16);
+  }
 };
+template <int L>
+__constant__ HermDecode<L> kHermDecode = HermDecode<L>();
 
+// E^{ij}_t of a table built by hermite_e<LA, LB>.
 template <int LA, int LB, typename Real>
 __device__ inline Real eidx(const Real* e, int i, int j, int t) {
   return e[(i * (LB + 1) + j) * (LA + LB + 1) + t];
 }
 
+// Largest number of Hermite terms (t, u, v) of one Cartesian pair of shells
+// (la, lb): prod over x, y, z of (a_x + b_x + 1).
+__host__ __device__ constexpr int pair_terms(int la, int lb) {
+  int m = 1;
+  for (int a = 0; a < ncart(la); ++a)
+    for (int b = 0; b < ncart(lb); ++b) {
+      const int sx = cart_lx(la, a) + cart_lx(lb, b), sy = cart_ly(la, a) + cart_ly(lb, b);
+      const int sz = la + lb - sx - sy;
+      const int n = (sx + 1) * (sy + 1) * (sz + 1);
+      m = n > m ? n : m;
+    }
+  return m;
+}
+
+constexpr int EW_BUDGET = 6144;     // 8-byte words of LDS per wave
+constexpr int EW_KC_BUDGET = 2560;  // batching ket primitives stops here (occupancy)
+
+// One 64-lane wave per contracted quartet.  Per bra primitive pair: the bra
+// E tables and, per bra Cartesian pair ab, the list of its nonzero
+// E^{ab}_{tuv} (coefficient, Hermite index).  Per batch of ket primitive
+// pairs: ket E tables, (-2 alpha)^n F_n, the R^0_tuv box (R recursion level
+// by level), per ket Cartesian pair cd the list of (pref (-1)^{t'+u'+v'}
+// E^{cd}_{t'u'v'}, box offset) terms, then h_cd[tuv] = sum_terms c R[tuv + t'u'v']
+// (items (cd, tuv) over all lanes), then every lane adds to the Cartesian
+// block elements it owns: V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv].
+template <int LA, int LB, int LC, int LD, typename Real>
+struct EriWave {
+  static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
+  static constexpr int NB = ncart(LB), ND = ncart(LD);
+  static constexpr int NAB = ncart(LA) * NB, NCD = ncart(LC) * ND;
+  static constexpr int NHB = nherm(LAB), NHR = nherm(L);
+  static constexpr int EBS = (LA + 1) * (LB + 1) * (LAB + 1);
+  static constexpr int EKS = (LC + 1) * (LD + 1) * (LCD + 1);
+  static constexpr int RB = L + 1;
+  static constexpr int TB = pair_terms(LA, LB), TK = pair_terms(LC, LD);
+  static constexpr int IPL = (NAB * NCD + 63) / 64;  // block elements per lane
+  static constexpr int WR = sizeof(Real) == 8 ? 2 : 1;  // Real per 8-byte word, inverse
+  // 8-byte words: fixed, per ket primitive of a batch
+  static constexpr int FIX = (3 * EBS + NAB * TB) / (3 - WR) + (NAB * TB + NAB) / 2 + 4;
+  static constexpr int KPER = (3 * EKS + 2 * NHR + RB * RB * RB + (L + 1) + 4 + NCD * TK +
+                               NCD * NHB) / (3 - WR) + (NCD * TK + NCD) / 2 + 2;
+  static constexpr int BLK = NAB * NCD / (3 - WR) + 1;
+  static constexpr int lds(int kc) { return FIX + (kc * KPER > BLK ? kc * KPER : BLK); }
+  static constexpr int pick_kc() {
+    int kc = COOP_MAXP;
+    while (kc > 1 && lds(kc) > EW_KC_BUDGET) --kc;
+    return kc;
+  }
+  static constexpr int KC = pick_kc() >= 7 ? pick_kc() : 1;
+  struct PerKet {
+    Real ek[KC][3][EKS];
+    Real r[2][KC][NHR];
+    Real rc[KC][RB * RB * RB];
+    Real f[KC][L + 1];
+    Real misc[KC][4];  // X, Y, Z, pref
+    Real kc[KC][NCD][TK];
+    int kd[KC][NCD][TK];
+    int kn[KC][NCD];
+    Real hb[KC][NCD][NHB];
+  };
+  struct Shared {
+    Real eb[3][EBS];
+    Real bc[NAB][TB];
+    int bi[NAB][TB];
+    int bn[NAB];
+    union {
+      PerKet k;
+      Real blk[NAB * NCD];  // the contracted Cartesian block, after the primitive loops
+    } u;
+  };
+};
+
 template <int LA, int LB, int LC, int LD, typename Real>
 __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
-  using S = CoopShared<LA, LB, LC, LD, Real>;
-  constexpr int LAB = S::LAB, LCD = S::LCD, L = S::L;
+  using W = EriWave<LA, LB, LC, LD, Real>;
+  constexpr int LAB = W::LAB, LCD = W::LCD, L = W::L;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-  constexpr int NAB = S::NAB, NCD = S::NCD, NHL = S::NHL, ESA = S::ESA, ESC = S::ESC;
-  constexpr int G = 64 / NCD;  // lanes per ket Cartesian pair
+  constexpr int NAB = W::NAB, NCD = W::NCD, NHB = W::NHB, NHR = W::NHR, RB = W::RB;
+  constexpr int TB = W::TB, TK = W::TK, KC = W::KC, IPL = W::IPL;
   const HermTable<L>& HT = kHermTable<L>;
-  __shared__ S sh;
+  __shared__ typename W::Shared sh;
+  auto& K = sh.u.k;
 
   const int lane = threadIdx.x;
   int bp, kp;
@@ -103,53 +174,62 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
   const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
   const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
 
-  const int cd = lane % NCD, g = lane / NCD;
-  const bool active = g < G;
-  const int c = cd / ND, d = cd % ND;
-  const int cx = CX[LC][c] , cy = CY[LC][c], cz = CZ[LC][c];
-  const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
-  const int sx = cx + dx, sy = cy + dy, sz = cz + dz;
-
-  Real acc[NAB];
+  Real acc[IPL];
 #pragma unroll
-  for (int i = 0; i < NAB; ++i) acc[i] = (Real)0;
+  for (int j = 0; j < IPL; ++j) acc[j] = (Real)0;
 
-  for (int ib0 = 0; ib0 < npb; ib0 += COOP_MAXP) {
-    const int nb = min(COOP_MAXP, npb - ib0);
+  for (int ib = 0; ib < npb; ++ib) {
+    const Real* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
     __syncthreads();
-    if (lane < nb) {
-      const Real* P = pp + (int64_t)(pb0 + ib0 + lane) * PP_STRIDE;
-      const Real oo2p = (Real)0.5 / P[PP_P];
+    if (lane < 3) {
       Real E[LA + 1][LB + 1][LAB + 1];
+      hermite_e<LA, LB, Real>(E, P[PP_PAX + lane], P[PP_PBX + lane], (Real)0.5 / P[PP_P]);
 #pragma unroll
-      for (int dim = 0; dim < 3; ++dim) {
-        hermite_e<LA, LB, Real>(E, P[PP_PAX + dim], P[PP_PBX + dim], oo2p);
+      for (int i = 0; i <= LA; ++i)
 #pragma unroll
-        for (int i = 0; i <= LA; ++i)
+        for (int j = 0; j <= LB; ++j)
 #pragma unroll
-          for (int j = 0; j <= LB; ++j)
-#pragma unroll
-            for (int t = 0; t <= LAB; ++t) sh.eb[lane][dim][(i * (LB + 1) + j) * (LAB + 1) + t] = E[i][j][t];
-      }
+          for (int t = 0; t <= LAB; ++t) sh.eb[lane][(i * (LB + 1) + j) * (LAB + 1) + t] = E[i][j][t];
     }
-    for (int ik = 0; ik < npk; ++ik) {
-      const Real* Q = pp + (int64_t)(pk0 + ik) * PP_STRIDE;
-      const Real q = Q[PP_P];
+    __syncthreads();
+    // bra term lists
+    for (int ab = lane; ab < NAB; ab += 64) {
+      const int a = ab / NB, b = ab - a * NB;
+      const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
+      const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
+      int n = 0;
+      for (int t = 0; t <= ax + bx; ++t) {
+        const Real ex = eidx<LA, LB, Real>(sh.eb[0], ax, bx, t);
+        for (int u = 0; u <= ay + by; ++u) {
+          const Real exy = ex * eidx<LA, LB, Real>(sh.eb[1], ay, by, u);
+          for (int v = 0; v <= az + bz; ++v) {
+            sh.bc[ab][n] = exy * eidx<LA, LB, Real>(sh.eb[2], az, bz, v);
+            sh.bi[ab][n] = herm_index(t, u, v);
+            ++n;
+          }
+        }
+      }
+      sh.bn[ab] = n;
+    }
+    for (int ik0 = 0; ik0 < npk; ik0 += KC) {
+      const int nk = min(KC, npk - ik0);
       __syncthreads();
-      if (lane < 3) {
+      if (lane < 3 * nk) {
+        const int kk = lane / 3, dim = lane - 3 * kk;
+        const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
         Real E[LC + 1][LD + 1][LCD + 1];
-        hermite_e<LC, LD, Real>(E, Q[PP_PAX + lane], Q[PP_PBX + lane], (Real)0.5 / q);
+        hermite_e<LC, LD, Real>(E, Q[PP_PAX + dim], Q[PP_PBX + dim], (Real)0.5 / Q[PP_P]);
 #pragma unroll
         for (int i = 0; i <= LC; ++i)
 #pragma unroll
           for (int j = 0; j <= LD; ++j)
 #pragma unroll
-            for (int t = 0; t <= LCD; ++t) sh.ek[lane][(i * (LD + 1) + j) * (LCD + 1) + t] = E[i][j][t];
+            for (int t = 0; t <= LCD; ++t) K.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 1) + t] = E[i][j][t];
       }
-      if (lane >= 32 && lane - 32 < nb) {
-        const int ib = lane - 32;
-        const Real* P = pp + (int64_t)(pb0 + ib0 + ib) * PP_STRIDE;
-        const Real p = P[PP_P];
+      if (lane >= 32 && lane - 32 < nk) {
+        const int kk = lane - 32;
+        const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
+        const Real p = P[PP_P], q = Q[PP_P];
         const Real pqm = p * q, ps = p + q, alpha = pqm / ps;
         const Real X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
                    Z = P[PP_PX + 2] - Q[PP_PX + 2];
@@ -158,117 +238,95 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
         Real pw = (Real)1;
 #pragma unroll
         for (int n = 0; n <= L; ++n) {
-          sh.f[ib][n] = pw * F[n];
+          K.f[kk][n] = pw * F[n];
           pw *= (Real)-2 * alpha;
         }
-        sh.pq[ib][0] = X;
-        sh.pq[ib][1] = Y;
-        sh.pq[ib][2] = Z;
-        sh.pq[ib][3] = (Real)34.986836655249725693 / (pqm * dev_sqrt(ps)) * P[PP_K] * Q[PP_K];
+        K.misc[kk][0] = X;
+        K.misc[kk][1] = Y;
+        K.misc[kk][2] = Z;
+        K.misc[kk][3] = (Real)34.986836655249725693 / (pqm * dev_sqrt(ps)) * P[PP_K] * Q[PP_K];
       }
       __syncthreads();
-      // stage 1: R^n for n = L .. 0, level n in buffer n & 1
+      // ket term lists (pref and sign folded in)
+      for (int w = lane; w < nk * NCD; w += 64) {
+        const int kk = w / NCD, cd = w - kk * NCD;
+        const int c = cd / ND, d = cd - c * ND;
+        const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
+        const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
+        const Real pref = K.misc[kk][3];
+        int n = 0;
+        for (int tt = 0; tt <= cx + dx; ++tt) {
+          const Real ex = eidx<LC, LD, Real>(K.ek[kk][0], cx, dx, tt);
+          for (int uu = 0; uu <= cy + dy; ++uu) {
+            const Real exy = ex * eidx<LC, LD, Real>(K.ek[kk][1], cy, dy, uu);
+            for (int vv = 0; vv <= cz + dz; ++vv) {
+              const Real e = pref * exy * eidx<LC, LD, Real>(K.ek[kk][2], cz, dz, vv);
+              K.kc[kk][cd][n] = ((tt + uu + vv) & 1) ? -e : e;
+              K.kd[kk][cd][n] = (tt * RB + uu) * RB + vv;
+              ++n;
+            }
+          }
+        }
+        K.kn[kk][cd] = n;
+      }
+      // R^0_tuv (t+u+v <= L) per ket primitive of the batch, levels n = L
+      // .. 0; level 0 also into the RB^3 box
 #pragma unroll
       for (int n = L; n >= 0; --n) {
         const int nh = nherm(L - n);
-        Real* cur = sh.r + (n & 1) * COOP_MAXP * NHL;
-        const Real* prv = sh.r + ((n + 1) & 1) * COOP_MAXP * NHL;
-        for (int e = lane; e < nb * nh; e += 64) {
-          const int ib = e / nh, k = e - ib * nh;
+        for (int w = lane; w < nk * nh; w += 64) {
+          const int kk = w / nh, k = w - kk * nh;
+          Real* cur = K.r[n & 1][kk];
+          const Real* prv = K.r[(n + 1) & 1][kk];
           Real v;
           if (k == 0) {
-            v = sh.f[ib][n];
+            v = K.f[kk][n];
           } else {
             const HermRec hr = HT.rec[k];
-            v = sh.pq[ib][hr.dir] * prv[ib * NHL + hr.src1];
-            if (hr.src2 >= 0) v += (Real)hr.mult * prv[ib * NHL + hr.src2];
+            v = K.misc[kk][hr.dir] * prv[hr.src1];
+            if (hr.src2 >= 0) v += (Real)hr.mult * prv[hr.src2];
           }
-          cur[ib * NHL + k] = v;
+          cur[k] = v;
+          if (n == 0) {
+            const int tuv = kHermDecode<L>.tuv[k];
+            K.rc[kk][((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16)] = v;
+          }
         }
         __syncthreads();
       }
-      // stage 2
-      if (active) {
-        const Real* R0 = sh.r;  // level 0
-        for (int ib = g; ib < nb; ib += G) {
-          const Real* Rb = R0 + ib * NHL;
-          Real h[nherm(LAB)];
+      // h_cd[tuv] for t+u+v <= la+lb
+      for (int w = lane; w < nk * NCD * NHB; w += 64) {
+        const int kk = w / (NCD * NHB), r0 = w - kk * NCD * NHB;
+        const int cd = r0 / NHB, k = r0 - cd * NHB;
+        const int tuv = kHermDecode<LAB>.tuv[k];
+        const Real* Rb = K.rc[kk] + ((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16);
+        const int nn = min(K.kn[kk][cd], TK);
+        Real h = (Real)0;
+        for (int j = 0; j < nn; ++j) h += K.kc[kk][cd][j] * Rb[K.kd[kk][cd][j]];
+        K.hb[kk][cd][k] = h;
+      }
+      __syncthreads();
+      // V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv] for this lane's elements
 #pragma unroll
-          for (int t = 0; t <= LAB; ++t)
-#pragma unroll
-            for (int u = 0; u <= LAB - t; ++u)
-#pragma unroll
-              for (int v = 0; v <= LAB - t - u; ++v) {
-                Real s = (Real)0;
-                for (int tt = 0; tt <= sx; ++tt) {
-                  const Real ex = eidx<LC, LD, Real>(sh.ek[0], cx, dx, tt);
-                  for (int uu = 0; uu <= sy; ++uu) {
-                    const Real exy = ex * eidx<LC, LD, Real>(sh.ek[1], cy, dy, uu);
-                    for (int vv = 0; vv <= sz; ++vv) {
-                      const Real e = exy * eidx<LC, LD, Real>(sh.ek[2], cz, dz, vv);
-                      const Real r = Rb[herm_index(t + tt, u + uu, v + vv)];
-                      s += ((tt + uu + vv) & 1) ? -e * r : e * r;
-                    }
-                  }
-                }
-                h[herm_index(t, u, v)] = s;
-              }
-          const Real pref = sh.pq[ib][3];
-          const Real* ebx = sh.eb[ib][0];
-          const Real* eby = sh.eb[ib][1];
-          const Real* ebz = sh.eb[ib][2];
-#pragma unroll
-          for (int a = 0; a < NA; ++a)
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-              const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
-              const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
-              Real val = (Real)0;
-#pragma unroll
-              for (int t = 0; t <= LAB; ++t) {
-                if (t > ax + bx) continue;
-#pragma unroll
-                for (int u = 0; u <= LAB; ++u) {
-                  if (u > ay + by) continue;
-                  const Real exy = eidx<LA, LB, Real>(ebx, ax, bx, t) * eidx<LA, LB, Real>(eby, ay, by, u);
-#pragma unroll
-                  for (int v = 0; v <= LAB; ++v) {
-                    if (v > az + bz) continue;
-                    val += exy * eidx<LA, LB, Real>(ebz, az, bz, v) * h[herm_index(t, u, v)];
-                  }
-                }
-              }
-              acc[a * NB + b] += pref * val;
-            }
+      for (int j = 0; j < IPL; ++j) {
+        const int it = lane + 64 * j;
+        if (it < NAB * NCD) {
+          const int ab = it / NCD, cd = it - ab * NCD;
+          const int nb = min(sh.bn[ab], TB);
+          Real s = (Real)0;
+          for (int kk = 0; kk < nk; ++kk)
+            for (int m = 0; m < nb; ++m) s += sh.bc[ab][m] * K.hb[kk][cd][sh.bi[ab][m]];
+          acc[j] += s;
         }
       }
     }
   }
-  (void)ESA;
-  (void)ESC;
-
-  // sum the G partial blocks of each cd: red[lane][ab]
   __syncthreads();
-  Real* red = sh.r;
-  if (active) {
+  Real* blk = sh.u.blk;
 #pragma unroll
-    for (int i = 0; i < NAB; ++i) red[lane * NAB + i] = acc[i];
-  }
-  __syncthreads();
-  // Cartesian block blk[ab][cd] (NAB x NCD), built by lanes cd < NCD
-  Real tot[NAB];
-  if (lane < NCD) {
-#pragma unroll
-    for (int i = 0; i < NAB; ++i) tot[i] = red[lane * NAB + i];
-    for (int gg = 1; gg < G; ++gg)
-#pragma unroll
-      for (int i = 0; i < NAB; ++i) tot[i] += red[(gg * NCD + lane) * NAB + i];
-  }
-  __syncthreads();
-  Real* blk = sh.r;
-  if (lane < NCD) {
-#pragma unroll
-    for (int i = 0; i < NAB; ++i) blk[i * NCD + lane] = tot[i];
+  for (int j = 0; j < IPL; ++j) {
+    const int it = lane + 64 * j;
+    if (it < NAB * NCD) blk[it] = acc[j];
   }
   __syncthreads();
 
@@ -308,19 +366,19 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
       }
     }
     const double val = (double)v;
-    const int64_t I = ao_a + sa, J = ao_b + sb, K = ao_c + sc, Lx = ao_d + sd;
+    const int64_t I = ao_a + sa, J = ao_b + sb, Kx = ao_c + sc, Lx = ao_d + sd;
     if (args.mode == ERI_DENSE) {
       double* out = args.out;
-      out[((I * n + J) * n + K) * n + Lx] = val;
-      out[((J * n + I) * n + K) * n + Lx] = val;
-      out[((I * n + J) * n + Lx) * n + K] = val;
-      out[((J * n + I) * n + Lx) * n + K] = val;
-      out[((K * n + Lx) * n + I) * n + J] = val;
-      out[((K * n + Lx) * n + J) * n + I] = val;
-      out[((Lx * n + K) * n + I) * n + J] = val;
-      out[((Lx * n + K) * n + J) * n + I] = val;
+      out[((I * n + J) * n + Kx) * n + Lx] = val;
+      out[((J * n + I) * n + Kx) * n + Lx] = val;
+      out[((I * n + J) * n + Lx) * n + Kx] = val;
+      out[((J * n + I) * n + Lx) * n + Kx] = val;
+      out[((Kx * n + Lx) * n + I) * n + J] = val;
+      out[((Kx * n + Lx) * n + J) * n + I] = val;
+      out[((Lx * n + Kx) * n + I) * n + J] = val;
+      out[((Lx * n + Kx) * n + J) * n + I] = val;
     } else {
-      store_super(args.out + (args.sp_base ? args.sp_base[bp] : 0), I, J, K, Lx, val);
+      store_super(args.out + (args.sp_base ? args.sp_base[bp] : 0), I, J, Kx, Lx, val);
     }
   }
 }

@@ -611,17 +611,6 @@ hipError_t grad2e_launch(const Grad2eArgs& a, hipStream_t s) {
 // items (tuv, cd) are spread over all 64 lanes and ket pairs are taken in
 // chunks sized so that the LDS stays under GW_BUDGET doubles.
 // ---------------------------------------------------------------------------
-template <int L>
-struct HermDecode {
-  int tuv[nherm(L)];
-  constexpr HermDecode() : tuv() {
-    for (int s = 0; s <= L; ++s)
-      for (int t = s; t >= 0; --t)
-        for (int u = s - t; u >= 0; --u) tuv[herm_index(t, u, s - t - u)] = t | (u << 8) | ((s - t - u) << 16);
-  }
-};
-template <int L>
-__constant__ HermDecode<L> kHermDecode = HermDecode<L>();
 
 constexpr int GW_BUDGET = 6144;     // doubles of LDS per wave (48 KB)
 constexpr int GW_KC_BUDGET = 2560;  // batching ket primitives stops at 20 KB (8 waves per CU)
