@@ -121,6 +121,16 @@ int qc_unpack_fock(const double* H, const double* y, double scale, int64_t nbf, 
                    double* F, void* stream);
 
 /* ---- many molecules per launch (batched SCF) -------------------------- */
+/* qc_one_electron for shell pairs of many molecules: pair sp writes its
+ * block at M + pair_base[sp] (row stride nmax, i.e. a padded
+ * [nmol][nmax][nmax] tensor) with molecule-local AO indices in sp_ao, and
+ * (which = 2) sums the nuclei pair_atoms[2 sp] .. + pair_atoms[2 sp + 1] of
+ * the concatenated atom arrays. */
+int qc_one_electron_batch(int which, int cab, const double* pairs64, const int* sp_prim_off,
+                          const int* sp_nprim, const int* sp_ao, const int* pair_list, int npairs,
+                          const int* pair_atoms, const double* atom_xyz, const double* atom_z,
+                          int64_t nmax, const int64_t* pair_base, double* M, void* stream);
+
 /* Each molecule m owns a tile-major supermatrix of molT[m] tile rows; the
  * molecules' tiles are numbered consecutively (first global tile tile0[m]),
  * as are their packed-vector rows (first row row0[m], 64 entries per row).

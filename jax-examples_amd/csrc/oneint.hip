@@ -20,11 +20,16 @@ __global__ __launch_bounds__(128) void one_electron_kernel(
     int which, const double* __restrict__ pp, const int* __restrict__ sp_prim_off,
     const int* __restrict__ sp_nprim, const int* __restrict__ sp_ao, const int* __restrict__ list,
     int npairs, int natom, const double* __restrict__ axyz, const double* __restrict__ az,
-    int64_t nbf, double* __restrict__ M, const double* __restrict__ boys) {
+    int64_t nbf, double* __restrict__ M, const double* __restrict__ boys,
+    const int* __restrict__ pair_atoms, const int64_t* __restrict__ pair_base) {
   constexpr int NA = ncart(LA), NB = ncart(LB), L = LA + LB;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= npairs) return;
   const int sp = list[idx];
+  // many molecules in one launch: the pair's own nuclei and output block
+  const int c0 = pair_atoms ? pair_atoms[2 * sp] : 0;
+  const int c1 = pair_atoms ? c0 + pair_atoms[2 * sp + 1] : natom;
+  if (pair_base) M += pair_base[sp];
   const int p0 = sp_prim_off[sp], np = sp_nprim[sp];
   double acc[NA][NB];
 #pragma unroll
@@ -41,7 +46,7 @@ __global__ __launch_bounds__(128) void one_electron_kernel(
       hermite_e<LA, LB, double>(Ex, P[PP_PAX], P[PP_PBX], oo2p);
       hermite_e<LA, LB, double>(Ey, P[PP_PAX + 1], P[PP_PBX + 1], oo2p);
       hermite_e<LA, LB, double>(Ez, P[PP_PAX + 2], P[PP_PBX + 2], oo2p);
-      for (int c = 0; c < natom; ++c) {
+      for (int c = c0; c < c1; ++c) {
         const double X = P[PP_PX] - axyz[3 * c], Y = P[PP_PX + 1] - axyz[3 * c + 1],
                      Z = P[PP_PX + 2] - axyz[3 * c + 2];
         double F[L + 1];
@@ -156,10 +161,10 @@ __global__ __launch_bounds__(128) void one_electron_kernel(
 
 using namespace qcmd;
 
-extern "C" int qc_one_electron(int which, int cab, const double* pairs64, const int* sp_prim_off,
-                               const int* sp_nprim, const int* sp_ao, const int* pair_list,
-                               int npairs, int natom, const double* atom_xyz, const double* atom_z,
-                               int64_t nbf, double* M, void* stream) {
+static int one_electron(int which, int cab, const double* pairs64, const int* sp_prim_off,
+                        const int* sp_nprim, const int* sp_ao, const int* pair_list, int npairs,
+                        int natom, const double* atom_xyz, const double* atom_z, int64_t nbf,
+                        double* M, const int* pair_atoms, const int64_t* pair_base, void* stream) {
   if (which < 0 || which > 2) return set_error("qc_one_electron: which=%d", which);
   if (npairs <= 0) return 0;
   const double* boys = boys_table64();
@@ -170,7 +175,7 @@ extern "C" int qc_one_electron(int which, int cab, const double* pairs64, const 
   case (A) * ((A) + 1) / 2 + (B):                                                             \
     hipLaunchKernelGGL((one_electron_kernel<A, B>), grid, block, 0, s, which, pairs64,        \
                        sp_prim_off, sp_nprim, sp_ao, pair_list, npairs, natom, atom_xyz, atom_z, \
-                       nbf, M, boys);                                                         \
+                       nbf, M, boys, pair_atoms, pair_base);                                  \
     break;
   switch (cab) {
     QC_ONE(0, 0) QC_ONE(1, 0) QC_ONE(1, 1) QC_ONE(2, 0) QC_ONE(2, 1) QC_ONE(2, 2)
@@ -179,4 +184,23 @@ extern "C" int qc_one_electron(int which, int cab, const double* pairs64, const 
   }
 #undef QC_ONE
   return check(hipGetLastError(), "qc_one_electron launch");
+}
+
+extern "C" int qc_one_electron(int which, int cab, const double* pairs64, const int* sp_prim_off,
+                               const int* sp_nprim, const int* sp_ao, const int* pair_list,
+                               int npairs, int natom, const double* atom_xyz, const double* atom_z,
+                               int64_t nbf, double* M, void* stream) {
+  return one_electron(which, cab, pairs64, sp_prim_off, sp_nprim, sp_ao, pair_list, npairs, natom,
+                      atom_xyz, atom_z, nbf, M, nullptr, nullptr, stream);
+}
+
+extern "C" int qc_one_electron_batch(int which, int cab, const double* pairs64,
+                                     const int* sp_prim_off, const int* sp_nprim, const int* sp_ao,
+                                     const int* pair_list, int npairs, const int* pair_atoms,
+                                     const double* atom_xyz, const double* atom_z, int64_t nmax,
+                                     const int64_t* pair_base, double* M, void* stream) {
+  if (!pair_base || (which == 2 && !pair_atoms))
+    return set_error("qc_one_electron_batch: null pair_base or pair_atoms");
+  return one_electron(which, cab, pairs64, sp_prim_off, sp_nprim, sp_ao, pair_list, npairs, 0,
+                      atom_xyz, atom_z, nmax, M, pair_atoms, pair_base, stream);
 }

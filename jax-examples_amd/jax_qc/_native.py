@@ -74,6 +74,9 @@ _SIGS = {
                   c_int, c_int, c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p,
                   c_void_p],
     "qc_tiles_maxabs": [c_void_p, c_i64, c_void_p, c_void_p],
+    "qc_one_electron_batch": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_int, c_void_p, c_void_p, c_void_p, c_i64, c_void_p, c_void_p,
+                              c_void_p],
     "qc_tiles_pack": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p, c_void_p],
     "qc_batch_symv_tiles": [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_i64, c_i64, c_i64, c_int, c_int, c_void_p, c_void_p,

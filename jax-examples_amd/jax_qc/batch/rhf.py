@@ -32,12 +32,7 @@ import torch
 from jax_qc import _native as nat
 from jax_qc.basis import build_basis_set
 from jax_qc.integrals import engine as eng
-from jax_qc.integrals.one_electron import (
-    compute_kinetic_matrix,
-    compute_nuclear_matrix,
-    compute_overlap_matrix,
-    nuclear_repulsion_energy,
-)
+from jax_qc.integrals.one_electron import nuclear_repulsion_energy
 from jax_qc.integrals.store import TILE_F32, TILE_F64, TILE_ZERO
 
 PAD_SHIFT = 1.0e4  # Eh on the padded diagonal of H (above any orbital energy here)
@@ -203,16 +198,12 @@ class BatchedRHF:
         self.n_occ = np.array([m.n_electrons // 2 for m in self.mols], dtype=np.int64)
         self.E_nuc = np.array([nuclear_repulsion_energy(m) for m in self.mols])
         n, N = self.nmol, self.nmax
-        S = torch.zeros(n, N, N, dtype=torch.float64, device=self.dev)
-        H = torch.zeros_like(S)
+        S, T, V = self._one_electron()
         pad = torch.zeros(n, N, dtype=torch.float64, device=self.dev)
-        for i, (mol, b, t) in enumerate(zip(self.mols, self.bases, tables)):
-            k = int(self.nbf_m[i])
-            S[i, :k, :k] = compute_overlap_matrix(b, t)
-            H[i, :k, :k] = compute_kinetic_matrix(b, t) + compute_nuclear_matrix(b, mol, t)
-            pad[i, k:] = 1.0
+        for i in range(n):
+            pad[i, int(self.nbf_m[i]):] = 1.0
         self.S = S + torch.diag_embed(pad)
-        self.H = H + PAD_SHIFT * torch.diag_embed(pad)
+        self.H = T + V + PAD_SHIFT * torch.diag_embed(pad)
         self.one_electron_seconds = time.perf_counter() - t0
 
         # ---- ERIs: one launch per class pair for every molecule ----------
@@ -243,6 +234,35 @@ class BatchedRHF:
         self.assembly_seconds = time.perf_counter() - t0 - self.eri_seconds
         self._setup_fock(seg)
         self.graph = None
+
+    def _one_electron(self):
+        """S, T, V of every molecule, padded to nmax: one launch per pair
+        class and matrix for all molecules (qc_one_electron_batch)."""
+        bpt, dev, N = self.bpt, self.dev, self.nmax
+        nat_m = np.array([len(m.atomic_numbers) for m in self.mols], dtype=np.int64)
+        at0 = np.concatenate([[0], np.cumsum(nat_m)[:-1]])
+        xyz = torch.from_numpy(np.ascontiguousarray(
+            np.concatenate([np.asarray(m.coords, dtype=np.float64) for m in self.mols]))).to(dev)
+        zz = torch.from_numpy(np.concatenate(
+            [np.asarray(m.atomic_numbers, dtype=np.float64) for m in self.mols])).to(dev)
+        mol = bpt.sp_mol
+        pair_atoms = torch.from_numpy(
+            np.stack([at0[mol], nat_m[mol]], 1).astype(np.int32).reshape(-1)).to(dev)
+        pair_base = torch.from_numpy(mol.astype(np.int64) * N * N).to(dev)
+        out = []
+        for which in (0, 1, 2):
+            M = torch.zeros(self.nmol, N, N, dtype=torch.float64, device=dev)
+            for c in range(6):
+                lst = bpt.class_pairs[c]
+                if len(lst) == 0:
+                    continue
+                ld = torch.from_numpy(lst).to(dev)
+                nat.call("qc_one_electron_batch", which, c, nat.ptr(bpt.pairs64),
+                         nat.ptr(bpt.sp_prim_off), nat.ptr(bpt.sp_nprim), nat.ptr(bpt.sp_ao),
+                         nat.ptr(ld), len(lst), nat.ptr(pair_atoms), nat.ptr(xyz), nat.ptr(zz), N,
+                         nat.ptr(pair_base), nat.ptr(M), nat.stream_ptr())
+            out.append(M)
+        return out
 
     def eri_launch(self, L: eng.ClassLaunch, J: torch.Tensor, variant: int, block: int = 0):
         """One class launch of the batch plan into the J pool ``J``."""
