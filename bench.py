@@ -31,6 +31,10 @@ for _p in (REPO, os.path.join(REPO, "jax-examples_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+# gradient of the mixed path: Schwarz 1e-11, density-weighted screening 1e-10,
+# primitive sums in FP32 for quartets with Schwarz bound below 1e-3 (the FP64
+# path: Schwarz 1e-12, nothing else)
+MIXED_GRAD = dict(grad_tau=1e-11, grad_tau_density=1e-10, grad_tau32=1e-3)
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
@@ -199,7 +203,9 @@ def _path(mol, bs, mode, scf, tau_f32_tile, report=False):
         torch.cuda.synchronize()
         t_scf = time.perf_counter() - t1
         t2 = time.perf_counter()
-        grad = scf_gradient(mol, bs, res, tau=1e-12, pair_table=store.pt)
+        gopt = (dict(tau=MIXED_GRAD["grad_tau"], tau_density=MIXED_GRAD["grad_tau_density"],
+                     tau32=MIXED_GRAD["grad_tau32"]) if mode == "mixed" else dict(tau=1e-12))
+        grad = scf_gradient(mol, bs, res, pair_table=store.pt, **gopt)
         torch.cuda.synchronize()
         t_grad = time.perf_counter() - t2
     return ints, store, store.P, res, t_ints, t_scf, grad, t_grad
@@ -314,9 +320,10 @@ def run_water16_uhf(args):
     """(H2O)_16 UHF triplet (grid 4x2x2, seed 2, even-tempered, 480
     functions): energy + analytic gradient through the plain FP64 path
     (Schwarz 1e-12, no other screening) and the mixed path (FP32-split ERIs
-    below 1e-4, FP32 supermatrix tiles, Schwarz 1e-11 and density-weighted
-    screening 1e-10 in the gradient), then BFGS steps on the mixed path with
-    the FP64 path evaluated at every accepted geometry."""
+    below 1e-4, FP32 supermatrix tiles, in the gradient Schwarz 1e-11,
+    density-weighted screening 1e-10 and FP32 primitive sums below 1e-3),
+    then BFGS steps on the mixed path with the FP64 path evaluated at every
+    accepted geometry."""
     import numpy as np
     import torch
 
@@ -328,8 +335,7 @@ def run_water16_uhf(args):
     mol = M.water_cluster((4, 2, 2), seed=2, spin=2)
     cfgc = CalcConfig(method="uhf", basis="even-tempered", scf_conv=1e-10, max_scf_iter=200)
     strict = dict(tau=1e-12, tau32=None, tile_f32=None, grad_tau=1e-12, grad_tau_density=0.0)
-    mixed = dict(tau=1e-12, tau32=1e-4, tile_f32=args.tile_f32, grad_tau=1e-11,
-                 grad_tau_density=1e-10)
+    mixed = dict(tau=1e-12, tau32=1e-4, tile_f32=args.tile_f32, **MIXED_GRAD)
 
     def timed(coords, opts):
         torch.cuda.synchronize()

@@ -201,10 +201,13 @@ typedef struct {
   double* grad;        /* natom x 3, accumulated (atomic adds) */
   int variant;         /* 0 library default, 1 one thread per quartet (low
                           classes only), 2 one wave per quartet with the bra
-                          side contracted with G first, 3 one wave per
-                          quartet, direct contraction */
+                          side contracted with G first */
   const double* schwarz; /* optional (with tau_density > 0): per shell pair Q */
   double tau_density;    /* skip a quartet when max|G_abcd| Q_ab Q_cd < tau_density */
+  const float* pairs32;  /* fp32 = 1: FP32 primitive-pair table */
+  int fp32;              /* 1: primitive sums in FP32 (wave kernel; classes whose default
+                            is one thread per quartet stay FP64), contributions summed
+                            and accumulated in FP64 */
 } qc_grad2e_args;
 
 /* 1/2 sum_quartets f sum_abcd G_abcd d(ab|cd)/dR for one class pair,

@@ -72,11 +72,11 @@ def _generate():
         path = os.path.join(GEN, f"grad2e_{tag}.hip")
         text = ('#include "grad_impl.h"\n'
                 "namespace qcmd {\n"
-                f"hipError_t grad2e_launch_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                f"  return grad2e_launch<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
-                "}\n"
                 f"hipError_t grad2e_launchw_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
+                "}\n"
+                f"hipError_t grad2e_launchw32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float>(a, s);\n"
                 "}\n"
                 f"hipError_t grad2e_launcht_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"

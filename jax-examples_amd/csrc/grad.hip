@@ -179,9 +179,9 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
 }
 
 #define QC_GDECL(A, B, C, D)                                                     \
-  hipError_t grad2e_launch_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);     \
   hipError_t grad2e_launcht_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);     \
-  hipError_t grad2e_launchw_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
+  hipError_t grad2e_launchw_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);   \
+  hipError_t grad2e_launchw32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
 QC_ERI_CLASSES(QC_GDECL)
 #undef QC_GDECL
 
@@ -247,20 +247,23 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   a.grad = in->grad;
   a.schwarz = in->schwarz;
   a.tau_density = in->tau_density;
+  a.pairs32 = in->pairs32;
+  a.boys32 = boys_table32();
+  if (in->fp32 && !a.pairs32) return set_error("qc_grad2e: fp32 launch without an FP32 pair table");
   if (a.tau_density > 0.0 && !a.schwarz) return set_error("qc_grad2e: tau_density without schwarz");
   a.boys64 = boys_table64();
   if (!a.boys64) return set_error("qc_grad2e: qc_init not called on the current device");
-  if (in->variant < 0 || in->variant > 3) return set_error("qc_grad2e: variant %d", in->variant);
+  if (in->variant < 0 || in->variant > 2) return set_error("qc_grad2e: variant %d", in->variant);
   // default (measured on (H2O)_8): one thread per quartet for ket (s s)
   // classes up to (dp|ss), one wave with the bra side contracted first for
   // the rest
   const int var = in->variant != 0 ? in->variant : (ccd == 0 && cab <= 4 ? 1 : 2);
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
-#define QC_GC(A, B, C, D)                                                  \
-  if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D)) \
-    e = var == 1 ? grad2e_launcht_##A##B##C##D(a, s)                        \
-                 : var == 2 ? grad2e_launchw_##A##B##C##D(a, s) : grad2e_launch_##A##B##C##D(a, s);
+#define QC_GC(A, B, C, D)                                                                     \
+  if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D))                    \
+    e = var == 1 ? grad2e_launcht_##A##B##C##D(a, s)                                           \
+                 : (in->fp32 ? grad2e_launchw32_##A##B##C##D(a, s) : grad2e_launchw_##A##B##C##D(a, s));
   QC_ERI_CLASSES(QC_GC)
 #undef QC_GC
   if (e == hipErrorNotSupported)

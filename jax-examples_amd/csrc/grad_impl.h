@@ -12,11 +12,10 @@
 // This is the contraction the reference's rhf_gradient forms by finite
 // differences of full integral rebuilds (jax_qc/grad/rhf_grad.py:153-209).
 //
-// Layout: one 64-lane wave per contracted quartet, as eri_coop_kernel:
-// stage 1 builds R^0_tuv up to L+1 for the bra primitive pairs of a chunk
-// (levels in LDS); stage 2: lane (cd, g) owns ket pair cd and bra
-// primitives g, g+G, ...; 9 accumulators (A, B, C x 3) per lane, summed
-// over the wave at the end and added atomically into grad[atom][xyz].
+// Two kernel forms (chosen per class): one thread per contracted quartet,
+// everything in registers (ket (s s) classes), and one 64-lane wave per
+// quartet with the bra side contracted with G first (the rest); both add the
+// quartet's A, B, C, D contributions atomically into grad[atom][xyz].
 #pragma once
 #include "eri_coop.h"
 
@@ -45,328 +44,24 @@ struct Grad2eArgs {
   const double* boys64;
   const double* schwarz;  // per shell pair Q (density screening), or null
   double tau_density;
+  const float* pairs32;  // FP32 launches: primitive-pair table and Boys grid
+  const float* boys32;
 };
 
-constexpr int GMAXP = 9;
+template <typename Real>
+__device__ inline const Real* gpairs(const Grad2eArgs& a);
+template <>
+__device__ inline const double* gpairs<double>(const Grad2eArgs& a) { return a.pairs64; }
+template <>
+__device__ inline const float* gpairs<float>(const Grad2eArgs& a) { return a.pairs32; }
+template <typename Real>
+__device__ inline const Real* gboys(const Grad2eArgs& a);
+template <>
+__device__ inline const double* gboys<double>(const Grad2eArgs& a) { return a.boys64; }
+template <>
+__device__ inline const float* gboys<float>(const Grad2eArgs& a) { return a.boys32; }
 
-template <int LA, int LB, int LC, int LD>
-struct GradShared {
-  static constexpr int LAB = LA + LB, LCD = LC + LD, L1 = LAB + LCD + 1;
-  static constexpr int EA = (LA + 2) * (LB + 2) * (LAB + 3);
-  static constexpr int EC = (LC + 2) * (LD + 2) * (LCD + 3);
-  static constexpr int NHL = nherm(L1);
-  static constexpr int NAB = ncart(LA) * ncart(LB), NCD = ncart(LC) * ncart(LD);
-  double eb[GMAXP][3][EA];
-  double bexp[GMAXP][2];
-  double ek[3][EC];
-  double kexp[2];
-  double f[GMAXP][L1 + 1];
-  double pq[GMAXP][4];
-  double r[2 * GMAXP * NHL > 64 * 9 ? 2 * GMAXP * NHL : 64 * 9];
-  double gam[NAB * NCD];
-};
-
-// E[i][j][t] in a table built for (i <= LA+1, j <= LB+1, t <= LA+LB+2).
-template <int LA, int LB>
-__device__ __forceinline__ double ex(const double* e, int i, int j, int t) {
-  if (i < 0 || j < 0) return 0.0;
-  return e[(i * (LB + 2) + j) * (LA + LB + 3) + t];
-}
-
-template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(64) void grad2e_coop_kernel(Grad2eArgs args) {
-  using S = GradShared<LA, LB, LC, LD>;
-  constexpr int LAB = S::LAB, L1 = S::L1;
-  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-  constexpr int NAB = S::NAB, NCD = S::NCD, NHL = S::NHL, EA = S::EA, EC = S::EC;
-  constexpr int G = 64 / NCD;
-  const HermTable<L1>& HT = kHermTable<L1>;
-  __shared__ S sh;
-
-  const int lane = threadIdx.x;
-  // quartet -> (bra pair, ket pair) as in the ERI kernels
-  int lo = 0, hi = args.nbra - 1;
-  const int64_t gid = flat_block();
-  if (gid >= args.nquartets) return;  // whole work-group (one wave)
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (args.offsets[mid] <= gid)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const int bp = args.bra_list[lo];
-  const int kp = args.ket_list[args.kstart[lo] + (int)(gid - args.offsets[lo])];
-  const double* __restrict__ pp = args.pairs64;
-  const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
-  const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
-  const int64_t oa = args.sp_aoc[2 * bp], ob = args.sp_aoc[2 * bp + 1];
-  const int64_t oc = args.sp_aoc[2 * kp], od = args.sp_aoc[2 * kp + 1];
-  const int64_t nc = args.ncart;
-
-  // G_abcd for the whole block
-  for (int e = lane; e < NAB * NCD; e += 64) {
-    const int ab = e / NCD, cd = e % NCD;
-    const int64_t mu = oa + ab / NB, nu = ob + ab % NB, la = oc + cd / ND, si = od + cd % ND;
-    double g = args.dj[mu * nc + nu] * args.dj[la * nc + si];
-    double k = args.dk0[mu * nc + la] * args.dk0[nu * nc + si] +
-               args.dk0[mu * nc + si] * args.dk0[nu * nc + la];
-    if (args.nk > 1)
-      k += args.dk1[mu * nc + la] * args.dk1[nu * nc + si] +
-           args.dk1[mu * nc + si] * args.dk1[nu * nc + la];
-    sh.gam[e] = g - args.cx * k;
-  }
-
-  const int cd = lane % NCD, g = lane / NCD;
-  const bool active = g < G;
-  const int c = cd / ND, d = cd % ND;
-  const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
-  const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
-  const int sx = cx + dx, sy = cy + dy, sz = cz + dz;
-
-  double acc[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) acc[i] = 0.0;
-
-  for (int ib0 = 0; ib0 < npb; ib0 += GMAXP) {
-    const int nb = min(GMAXP, npb - ib0);
-    __syncthreads();
-    if (lane < nb) {
-      const double* P = pp + (int64_t)(pb0 + ib0 + lane) * PP_STRIDE;
-      const double oo2p = 0.5 / P[PP_P];
-      double E[LA + 2][LB + 2][LAB + 3];
-#pragma unroll
-      for (int dim = 0; dim < 3; ++dim) {
-        hermite_e<LA + 1, LB + 1, double>(E, P[PP_PAX + dim], P[PP_PBX + dim], oo2p);
-#pragma unroll
-        for (int i = 0; i <= LA + 1; ++i)
-#pragma unroll
-          for (int j = 0; j <= LB + 1; ++j)
-#pragma unroll
-            for (int t = 0; t <= LAB + 2; ++t)
-              sh.eb[lane][dim][(i * (LB + 2) + j) * (LAB + 3) + t] = E[i][j][t];
-      }
-      sh.bexp[lane][0] = P[PP_ALPHA];
-      sh.bexp[lane][1] = P[PP_P] - P[PP_ALPHA];
-    }
-    for (int ik = 0; ik < npk; ++ik) {
-      const double* Q = pp + (int64_t)(pk0 + ik) * PP_STRIDE;
-      const double q = Q[PP_P];
-      __syncthreads();
-      if (lane < 3) {
-        double E[LC + 2][LD + 2][S::LCD + 3];
-        hermite_e<LC + 1, LD + 1, double>(E, Q[PP_PAX + lane], Q[PP_PBX + lane], 0.5 / q);
-#pragma unroll
-        for (int i = 0; i <= LC + 1; ++i)
-#pragma unroll
-          for (int j = 0; j <= LD + 1; ++j)
-#pragma unroll
-            for (int t = 0; t <= S::LCD + 2; ++t)
-              sh.ek[lane][(i * (LD + 2) + j) * (S::LCD + 3) + t] = E[i][j][t];
-        if (lane == 0) {
-          sh.kexp[0] = Q[PP_ALPHA];
-          sh.kexp[1] = q - Q[PP_ALPHA];
-        }
-      }
-      if (lane >= 32 && lane - 32 < nb) {
-        const int ib = lane - 32;
-        const double* P = pp + (int64_t)(pb0 + ib0 + ib) * PP_STRIDE;
-        const double p = P[PP_P];
-        const double pqm = p * q, ps = p + q, alpha = pqm / ps;
-        const double X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
-                     Z = P[PP_PX + 2] - Q[PP_PX + 2];
-        double F[L1 + 1];
-        boys_fn<L1, double>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
-        double pw = 1.0;
-#pragma unroll
-        for (int n = 0; n <= L1; ++n) {
-          sh.f[ib][n] = pw * F[n];
-          pw *= -2.0 * alpha;
-        }
-        sh.pq[ib][0] = X;
-        sh.pq[ib][1] = Y;
-        sh.pq[ib][2] = Z;
-        sh.pq[ib][3] = 34.986836655249725693 / (pqm * sqrt(ps)) * P[PP_K] * Q[PP_K];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int n = L1; n >= 0; --n) {
-        const int nh = nherm(L1 - n);
-        double* cur = sh.r + (n & 1) * GMAXP * NHL;
-        const double* prv = sh.r + ((n + 1) & 1) * GMAXP * NHL;
-        for (int e = lane; e < nb * nh; e += 64) {
-          const int ib = e / nh, k = e - ib * nh;
-          double v;
-          if (k == 0) {
-            v = sh.f[ib][n];
-          } else {
-            const HermRec hr = HT.rec[k];
-            v = sh.pq[ib][hr.dir] * prv[ib * NHL + hr.src1];
-            if (hr.src2 >= 0) v += (double)hr.mult * prv[ib * NHL + hr.src2];
-          }
-          cur[ib * NHL + k] = v;
-        }
-        __syncthreads();
-      }
-      if (active) {
-        const double ga2 = 2.0 * sh.kexp[0];
-        const double* ekx = sh.ek[0];
-        const double* eky = sh.ek[1];
-        const double* ekz = sh.ek[2];
-        for (int ib = g; ib < nb; ib += G) {
-          const double* Rb = sh.r + ib * NHL;
-          const double al2 = 2.0 * sh.bexp[ib][0], be2 = 2.0 * sh.bexp[ib][1];
-          const double pref = sh.pq[ib][3];
-          const double* ebx = sh.eb[ib][0];
-          const double* eby = sh.eb[ib][1];
-          const double* ebz = sh.eb[ib][2];
-          // h_tuv, t+u+v <= LAB+1 (ket undifferentiated)
-          double h[nherm(LAB + 1)];
-#pragma unroll
-          for (int t = 0; t <= LAB + 1; ++t)
-#pragma unroll
-            for (int u = 0; u <= LAB + 1 - t; ++u)
-#pragma unroll
-              for (int v = 0; v <= LAB + 1 - t - u; ++v) {
-                double s = 0.0;
-                for (int tt = 0; tt <= sx; ++tt) {
-                  const double e1 = ex<LC, LD>(ekx, cx, dx, tt);
-                  for (int uu = 0; uu <= sy; ++uu) {
-                    const double e2 = e1 * ex<LC, LD>(eky, cy, dy, uu);
-                    for (int vv = 0; vv <= sz; ++vv) {
-                      const double e3 = e2 * ex<LC, LD>(ekz, cz, dz, vv);
-                      const double r = Rb[herm_index(t + tt, u + uu, v + vv)];
-                      s += ((tt + uu + vv) & 1) ? -e3 * r : e3 * r;
-                    }
-                  }
-                }
-                h[herm_index(t, u, v)] = s;
-              }
-#pragma unroll
-          for (int a = 0; a < NA; ++a)
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-              const double gm = sh.gam[(a * NB + b) * NCD + cd] * pref;
-              const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
-              const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
-              double sAx = 0, sAy = 0, sAz = 0, sBx = 0, sBy = 0, sBz = 0;
-#pragma unroll
-              for (int t = 0; t <= LAB + 1; ++t) {
-                if (t > ax + bx + 1) continue;
-                const double e0x = ex<LA, LB>(ebx, ax, bx, t);
-                const double eAx = al2 * ex<LA, LB>(ebx, ax + 1, bx, t) - ax * ex<LA, LB>(ebx, ax - 1, bx, t);
-                const double eBx = be2 * ex<LA, LB>(ebx, ax, bx + 1, t) - bx * ex<LA, LB>(ebx, ax, bx - 1, t);
-#pragma unroll
-                for (int u = 0; u <= LAB + 1; ++u) {
-                  if (u > ay + by + 1 || t + u > LAB + 1) continue;
-                  const double e0y = ex<LA, LB>(eby, ay, by, u);
-                  const double eAy = al2 * ex<LA, LB>(eby, ay + 1, by, u) - ay * ex<LA, LB>(eby, ay - 1, by, u);
-                  const double eBy = be2 * ex<LA, LB>(eby, ay, by + 1, u) - by * ex<LA, LB>(eby, ay, by - 1, u);
-#pragma unroll
-                  for (int v = 0; v <= LAB + 1; ++v) {
-                    if (v > az + bz + 1 || t + u + v > LAB + 1) continue;
-                    const double e0z = ex<LA, LB>(ebz, az, bz, v);
-                    const double eAz = al2 * ex<LA, LB>(ebz, az + 1, bz, v) - az * ex<LA, LB>(ebz, az - 1, bz, v);
-                    const double eBz = be2 * ex<LA, LB>(ebz, az, bz + 1, v) - bz * ex<LA, LB>(ebz, az, bz - 1, v);
-                    const double hv = h[herm_index(t, u, v)];
-                    sAx += eAx * e0y * e0z * hv;
-                    sAy += e0x * eAy * e0z * hv;
-                    sAz += e0x * e0y * eAz * hv;
-                    sBx += eBx * e0y * e0z * hv;
-                    sBy += e0x * eBy * e0z * hv;
-                    sBz += e0x * e0y * eBz * hv;
-                  }
-                }
-              }
-              acc[0] += gm * sAx;
-              acc[1] += gm * sAy;
-              acc[2] += gm * sAz;
-              acc[3] += gm * sBx;
-              acc[4] += gm * sBy;
-              acc[5] += gm * sBz;
-            }
-          // C derivatives, one Cartesian direction at a time
-#pragma unroll
-          for (int dim = 0; dim < 3; ++dim) {
-            double hc[nherm(LAB)];
-            const int ex2 = sx + (dim == 0), ey2 = sy + (dim == 1), ez2 = sz + (dim == 2);
-#pragma unroll
-            for (int t = 0; t <= LAB; ++t)
-#pragma unroll
-              for (int u = 0; u <= LAB - t; ++u)
-#pragma unroll
-                for (int v = 0; v <= LAB - t - u; ++v) {
-                  double s = 0.0;
-                  for (int tt = 0; tt <= ex2; ++tt) {
-                    const double e1 = dim == 0 ? ga2 * ex<LC, LD>(ekx, cx + 1, dx, tt) -
-                                                     cx * ex<LC, LD>(ekx, cx - 1, dx, tt)
-                                               : ex<LC, LD>(ekx, cx, dx, tt);
-                    for (int uu = 0; uu <= ey2; ++uu) {
-                      const double e2 = e1 * (dim == 1 ? ga2 * ex<LC, LD>(eky, cy + 1, dy, uu) -
-                                                             cy * ex<LC, LD>(eky, cy - 1, dy, uu)
-                                                       : ex<LC, LD>(eky, cy, dy, uu));
-                      for (int vv = 0; vv <= ez2; ++vv) {
-                        const double e3 = e2 * (dim == 2 ? ga2 * ex<LC, LD>(ekz, cz + 1, dz, vv) -
-                                                               cz * ex<LC, LD>(ekz, cz - 1, dz, vv)
-                                                         : ex<LC, LD>(ekz, cz, dz, vv));
-                        const double r = Rb[herm_index(t + tt, u + uu, v + vv)];
-                        s += ((tt + uu + vv) & 1) ? -e3 * r : e3 * r;
-                      }
-                    }
-                  }
-                  hc[herm_index(t, u, v)] = s;
-                }
-#pragma unroll
-            for (int a = 0; a < NA; ++a)
-#pragma unroll
-              for (int b = 0; b < NB; ++b) {
-                const double gm = sh.gam[(a * NB + b) * NCD + cd] * pref;
-                const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
-                const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
-                double sC = 0.0;
-#pragma unroll
-                for (int t = 0; t <= LAB; ++t) {
-                  if (t > ax + bx) continue;
-#pragma unroll
-                  for (int u = 0; u <= LAB; ++u) {
-                    if (u > ay + by) continue;
-                    const double exy = ex<LA, LB>(ebx, ax, bx, t) * ex<LA, LB>(eby, ay, by, u);
-#pragma unroll
-                    for (int v = 0; v <= LAB; ++v) {
-                      if (v > az + bz) continue;
-                      sC += exy * ex<LA, LB>(ebz, az, bz, v) * hc[herm_index(t, u, v)];
-                    }
-                  }
-                }
-                acc[6 + dim] += gm * sC;
-              }
-          }
-        }
-      }
-    }
-  }
-  (void)EA;
-  (void)EC;
-  (void)NC;
-  // wave sum of the 9 accumulators
-#pragma unroll
-  for (int i = 0; i < 9; ++i)
-    for (int w = 32; w > 0; w >>= 1) acc[i] += __shfl_xor(acc[i], w);
-  if (lane == 0) {
-    const double f = (args.sp_diag[bp] ? 1.0 : 2.0) * (args.sp_diag[kp] ? 1.0 : 2.0) *
-                     (bp == kp ? 1.0 : 2.0) * 0.5;
-    const int A = args.sp_atom[2 * bp], B = args.sp_atom[2 * bp + 1];
-    const int C = args.sp_atom[2 * kp], D = args.sp_atom[2 * kp + 1];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double gA = f * acc[k], gB = f * acc[3 + k], gC = f * acc[6 + k];
-      atomicAdd(&args.grad[3 * A + k], gA);
-      atomicAdd(&args.grad[3 * B + k], gB);
-      atomicAdd(&args.grad[3 * C + k], gC);
-      atomicAdd(&args.grad[3 * D + k], -(gA + gB + gC));
-    }
-  }
-}
+constexpr int GMAXP = 9;  // ket primitive pairs per batch, at most
 
 // One thread per contracted quartet, for the low classes (the G block and
 // the 9 accumulators stay in registers).  Per primitive quartet: E tables up
@@ -584,14 +279,6 @@ hipError_t grad2e_launch_thread(const Grad2eArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int LA, int LB, int LC, int LD>
-hipError_t grad2e_launch(const Grad2eArgs& a, hipStream_t s) {
-  if (a.nquartets <= 0) return hipSuccess;
-  if (a.nquartets > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((grad2e_coop_kernel<LA, LB, LC, LD>), grid_1d(a.nquartets, 64), dim3(64), 0,
-                     s, a);
-  return hipGetLastError();
-}
 
 
 // ---------------------------------------------------------------------------
@@ -632,7 +319,7 @@ __host__ __device__ constexpr int tmax_terms(int lc, int ld) {
   return m;
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename Real>
 struct GradWave {
   static constexpr int LAB = LA + LB, LCD = LC + LD, L1 = LAB + LCD + 1;
   static constexpr int NB = ncart(LB), ND = ncart(LD);
@@ -644,12 +331,13 @@ struct GradWave {
   // most ket terms (t', u', v') of one Cartesian pair cd: list 0 for h, list
   // 1 + x for hC_x (one more term along x)
   static constexpr int TMAX = tmax_terms(LC, LD);
-  // LDS doubles: fixed part, per ket pair of a chunk, per ket primitive of
-  // a batch, per (ket pair, ket primitive)
-  static constexpr int FIX = NAB * NCD + 3 * EBS + 8;
-  static constexpr int YPER = NHB0 + 3 * NHB1;
-  static constexpr int KPER = 3 * EKS + 2 * NHR + RB * RB * RB + (L1 + 1) + 4;
-  static constexpr int TPER = 4 * TMAX * 3 / 2 + 4;
+  // LDS bytes: fixed part, per ket pair of a chunk, per ket primitive of a
+  // batch, per (ket pair, ket primitive)
+  static constexpr int RS = sizeof(Real);
+  static constexpr int FIX = (NAB * NCD + 3 * EBS) * RS + 64;
+  static constexpr int YPER = (NHB0 + 3 * NHB1) * RS;
+  static constexpr int KPER = (3 * EKS + 2 * NHR + RB * RB * RB + (L1 + 1) + 4) * RS;
+  static constexpr int TPER = 4 * TMAX * (RS + 4) + 16;
   static constexpr int lds(int nc, int kc) { return FIX + nc * YPER + kc * (KPER + nc * TPER); }
   // as many ket primitives per batch (at most GMAXP) as keep the whole ket
   // side within GW_KC_BUDGET (occupancy); ket pairs are then chunked to fit
@@ -657,7 +345,7 @@ struct GradWave {
   // term lists, R levels) among its primitive quartets.
   static constexpr int pick_kc() {
     int kc = GMAXP;
-    while (kc > 1 && lds(NCD, kc) > GW_KC_BUDGET) --kc;
+    while (kc > 1 && lds(NCD, kc) > 8 * GW_KC_BUDGET) --kc;
     return kc;
   }
   // batches of 2-6 cost more in occupancy than they save in barriers
@@ -665,29 +353,29 @@ struct GradWave {
   static constexpr int KC = pick_kc() >= 7 ? pick_kc() : 1;
   static constexpr int pick_nc() {
     int nc = NCD;
-    while (nc > 1 && lds(nc, KC) > GW_BUDGET) --nc;
+    while (nc > 1 && lds(nc, KC) > 8 * GW_BUDGET) --nc;
     return nc;
   }
   static constexpr int NCDC = pick_nc();
   struct Shared {
-    double gam[NAB * NCD];
-    double eb[3][EBS];
-    double y0[NCDC][NHB0];
-    double ya[3][NCDC][NHB1];
-    double ek[KC][3][EKS];
-    double r[2][KC][NHR];
-    double rc[KC][RB * RB * RB];
-    double f[KC][L1 + 1];
-    double misc[KC][4];           // X, Y, Z, pref
-    double tc[KC][4][NCDC][TMAX];  // term coefficients (sign, pref and derivative factors folded in)
+    Real gam[NAB * NCD];
+    Real eb[3][EBS];
+    Real y0[NCDC][NHB0];
+    Real ya[3][NCDC][NHB1];
+    Real ek[KC][3][EKS];
+    Real r[2][KC][NHR];
+    Real rc[KC][RB * RB * RB];
+    Real f[KC][L1 + 1];
+    Real misc[KC][4];           // X, Y, Z, pref
+    Real tc[KC][4][NCDC][TMAX];  // term coefficients (sign, pref and derivative factors folded in)
     int td[KC][4][NCDC][TMAX];     // term offsets in the R box
     int tn[KC][4][NCDC];           // term counts
   };
 };
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename Real>
 __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
-  using W = GradWave<LA, LB, LC, LD>;
+  using W = GradWave<LA, LB, LC, LD, Real>;
   constexpr int LAB = W::LAB, LCD = W::LCD, L1 = W::L1, NB = W::NB, ND = W::ND;
   constexpr int NAB = W::NAB, NCD = W::NCD, NHB0 = W::NHB0, NHB1 = W::NHB1, NHR = W::NHR;
   constexpr int NCDC = W::NCDC, KC = W::KC, RB = W::RB, TMAX = W::TMAX;
@@ -708,7 +396,7 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
   }
   const int bp = args.bra_list[lo];
   const int kp = args.ket_list[args.kstart[lo] + (int)(gid - args.offsets[lo])];
-  const double* __restrict__ pp = args.pairs64;
+  const Real* __restrict__ pp = gpairs<Real>(args);
   const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
   const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
   const int64_t oa = args.sp_aoc[2 * bp], ob = args.sp_aoc[2 * bp + 1];
@@ -723,34 +411,34 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
     if (args.nk > 1)
       k += args.dk1[mu * nc + la] * args.dk1[nu * nc + si] +
            args.dk1[mu * nc + si] * args.dk1[nu * nc + la];
-    sh.gam[e] = args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k;
+    sh.gam[e] = (Real)(args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k);
   }
   if (args.tau_density > 0.0) {
     // each lane's maximum over the G entries it wrote, then over the wave
     double gmax = 0.0;
-    for (int e = lane; e < NAB * NCD; e += 64) gmax = fmax(gmax, fabs(sh.gam[e]));
+    for (int e = lane; e < NAB * NCD; e += 64) gmax = fmax(gmax, fabs((double)sh.gam[e]));
     for (int w = 32; w > 0; w >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, w));
     if (gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return;  // wave-uniform
   }
 
   // E^{ij}_t of the bra / ket tables in LDS (zero for t > i + j).
-  auto eb = [&](int dim, int i, int j, int t) -> double {
-    return i < 0 ? 0.0 : sh.eb[dim][(i * (LB + 1) + j) * (LAB + 2) + t];
+  auto eb = [&](int dim, int i, int j, int t) -> Real {
+    return i < 0 ? (Real)0 : sh.eb[dim][(i * (LB + 1) + j) * (LAB + 2) + t];
   };
-  auto ek = [&](int kk, int dim, int i, int j, int t) -> double {
-    return i < 0 ? 0.0 : sh.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 2) + t];
+  auto ek = [&](int kk, int dim, int i, int j, int t) -> Real {
+    return i < 0 ? (Real)0 : sh.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 2) + t];
   };
 
-  double acc[9];  // dA xyz, dP xyz, dC xyz
+  Real acc[9];  // dA xyz, dP xyz, dC xyz
 #pragma unroll
-  for (int i = 0; i < 9; ++i) acc[i] = 0.0;
+  for (int i = 0; i < 9; ++i) acc[i] = (Real)0;
 
   for (int ib = 0; ib < npb; ++ib) {
-    const double* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
+    const Real* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
     __syncthreads();
     if (lane < 3) {
-      double E[LA + 2][LB + 1][LAB + 2];
-      hermite_e<LA + 1, LB, double>(E, P[PP_PAX + lane], P[PP_PBX + lane], 0.5 / P[PP_P]);
+      Real E[LA + 2][LB + 1][LAB + 2];
+      hermite_e<LA + 1, LB, Real>(E, P[PP_PAX + lane], P[PP_PBX + lane], (Real)0.5 / P[PP_P]);
 #pragma unroll
       for (int i = 0; i <= LA + 1; ++i)
 #pragma unroll
@@ -758,7 +446,7 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
 #pragma unroll
           for (int t = 0; t <= LAB + 1; ++t) sh.eb[lane][(i * (LB + 1) + j) * (LAB + 2) + t] = E[i][j][t];
     }
-    const double al2 = 2.0 * P[PP_ALPHA];
+    const Real al2 = (Real)2 * P[PP_ALPHA];
     for (int cd0 = 0; cd0 < NCD; cd0 += NCDC) {
       const int ncdc = min(NCDC, NCD - cd0);
       __syncthreads();
@@ -766,17 +454,17 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
       for (int it = lane; it < ncdc * NHB1; it += 64) {
         const int cl = it / NHB1, k = it - cl * NHB1, cd = cd0 + cl;
         const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
-        double y0 = 0.0, yx = 0.0, yy = 0.0, yz = 0.0;
+        Real y0 = (Real)0, yx = (Real)0, yy = (Real)0, yz = (Real)0;
         for (int ab = 0; ab < NAB; ++ab) {
           const int a = ab / NB, b = ab - a * NB;
           const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
           const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
           if (t > ax + bx + 1 || u > ay + by + 1 || v > az + bz + 1) continue;
-          const double g = sh.gam[ab * NCD + cd];
-          const double e0x = eb(0, ax, bx, t), e0y = eb(1, ay, by, u), e0z = eb(2, az, bz, v);
-          const double eax = al2 * eb(0, ax + 1, bx, t) - ax * eb(0, ax - 1, bx, t);
-          const double eay = al2 * eb(1, ay + 1, by, u) - ay * eb(1, ay - 1, by, u);
-          const double eaz = al2 * eb(2, az + 1, bz, v) - az * eb(2, az - 1, bz, v);
+          const Real g = sh.gam[ab * NCD + cd];
+          const Real e0x = eb(0, ax, bx, t), e0y = eb(1, ay, by, u), e0z = eb(2, az, bz, v);
+          const Real eax = al2 * eb(0, ax + 1, bx, t) - (Real)ax * eb(0, ax - 1, bx, t);
+          const Real eay = al2 * eb(1, ay + 1, by, u) - (Real)ay * eb(1, ay - 1, by, u);
+          const Real eaz = al2 * eb(2, az + 1, bz, v) - (Real)az * eb(2, az - 1, bz, v);
           y0 += g * e0x * e0y * e0z;
           yx += g * eax * e0y * e0z;
           yy += g * e0x * eay * e0z;
@@ -793,9 +481,9 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
         // ket E tables (lanes 3 kk + dim) and Boys values (lanes 32 + kk)
         if (lane < 3 * nk) {
           const int kk = lane / 3, dim = lane - 3 * kk;
-          const double* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
-          double E[LC + 2][LD + 1][LCD + 2];
-          hermite_e<LC + 1, LD, double>(E, Q[PP_PAX + dim], Q[PP_PBX + dim], 0.5 / Q[PP_P]);
+          const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
+          Real E[LC + 2][LD + 1][LCD + 2];
+          hermite_e<LC + 1, LD, Real>(E, Q[PP_PAX + dim], Q[PP_PBX + dim], (Real)0.5 / Q[PP_P]);
 #pragma unroll
           for (int i = 0; i <= LC + 1; ++i)
 #pragma unroll
@@ -805,23 +493,23 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
         }
         if (lane >= 32 && lane - 32 < nk) {
           const int kk = lane - 32;
-          const double* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
-          const double p = P[PP_P], q = Q[PP_P];
-          const double pqm = p * q, ps = p + q, alpha = pqm / ps;
-          const double X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
-                       Z = P[PP_PX + 2] - Q[PP_PX + 2];
-          double F[L1 + 1];
-          boys_fn<L1, double>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
-          double pw = 1.0;
+          const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
+          const Real p = P[PP_P], q = Q[PP_P];
+          const Real pqm = p * q, ps = p + q, alpha = pqm / ps;
+          const Real X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
+                     Z = P[PP_PX + 2] - Q[PP_PX + 2];
+          Real F[L1 + 1];
+          boys_fn<L1, Real>(alpha * (X * X + Y * Y + Z * Z), gboys<Real>(args), F);
+          Real pw = (Real)1;
 #pragma unroll
           for (int n = 0; n <= L1; ++n) {
             sh.f[kk][n] = pw * F[n];
-            pw *= -2.0 * alpha;
+            pw *= (Real)-2 * alpha;
           }
           sh.misc[kk][0] = X;
           sh.misc[kk][1] = Y;
           sh.misc[kk][2] = Z;
-          sh.misc[kk][3] = 34.986836655249725693 / (pqm * sqrt(ps)) * P[PP_K] * Q[PP_K];
+          sh.misc[kk][3] = (Real)34.986836655249725693 / (pqm * dev_sqrt(ps)) * P[PP_K] * Q[PP_K];
         }
         __syncthreads();
         // ket term lists per (ket primitive, list, ket pair): list 0 for h,
@@ -833,20 +521,20 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
           const int c = cd / ND, d = cd - c * ND;
           const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
           const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
-          const double pref = sh.misc[kk][3];
-          const double ga2 = 2.0 * pp[(int64_t)(pk0 + ik0 + kk) * PP_STRIDE + PP_ALPHA];
+          const Real pref = sh.misc[kk][3];
+          const Real ga2 = (Real)2 * pp[(int64_t)(pk0 + ik0 + kk) * PP_STRIDE + PP_ALPHA];
           const int dim = l - 1;
           const int ex2 = cx + dx + (dim == 0), ey2 = cy + dy + (dim == 1), ez2 = cz + dz + (dim == 2);
           int n = 0;
           for (int tt = 0; tt <= ex2; ++tt) {
-            const double e1 = dim == 0 ? ga2 * ek(kk, 0, cx + 1, dx, tt) - cx * ek(kk, 0, cx - 1, dx, tt)
-                                       : ek(kk, 0, cx, dx, tt);
+            const Real e1 = dim == 0 ? ga2 * ek(kk, 0, cx + 1, dx, tt) - (Real)cx * ek(kk, 0, cx - 1, dx, tt)
+                                     : ek(kk, 0, cx, dx, tt);
             for (int uu = 0; uu <= ey2; ++uu) {
-              const double e2 = e1 * (dim == 1 ? ga2 * ek(kk, 1, cy + 1, dy, uu) - cy * ek(kk, 1, cy - 1, dy, uu)
-                                               : ek(kk, 1, cy, dy, uu));
+              const Real e2 = e1 * (dim == 1 ? ga2 * ek(kk, 1, cy + 1, dy, uu) - (Real)cy * ek(kk, 1, cy - 1, dy, uu)
+                                             : ek(kk, 1, cy, dy, uu));
               for (int vv = 0; vv <= ez2; ++vv) {
-                const double e3 = e2 * (dim == 2 ? ga2 * ek(kk, 2, cz + 1, dz, vv) - cz * ek(kk, 2, cz - 1, dz, vv)
-                                                 : ek(kk, 2, cz, dz, vv));
+                const Real e3 = e2 * (dim == 2 ? ga2 * ek(kk, 2, cz + 1, dz, vv) - (Real)cz * ek(kk, 2, cz - 1, dz, vv)
+                                               : ek(kk, 2, cz, dz, vv));
                 sh.tc[kk][l][cl][n] = ((tt + uu + vv) & 1) ? -pref * e3 : pref * e3;
                 sh.td[kk][l][cl][n] = (tt * RB + uu) * RB + vv;
                 ++n;
@@ -863,15 +551,15 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
           const int nh = nherm(L1 - n);
           for (int w = lane; w < nk * nh; w += 64) {
             const int kk = w / nh, k = w - kk * nh;
-            double* cur = sh.r[n & 1][kk];
-            const double* prv = sh.r[(n + 1) & 1][kk];
-            double v;
+            Real* cur = sh.r[n & 1][kk];
+            const Real* prv = sh.r[(n + 1) & 1][kk];
+            Real v;
             if (k == 0) {
               v = sh.f[kk][n];
             } else {
               const HermRec hr = HT.rec[k];
               v = sh.misc[kk][hr.dir] * prv[hr.src1];
-              if (hr.src2 >= 0) v += (double)hr.mult * prv[hr.src2];
+              if (hr.src2 >= 0) v += (Real)hr.mult * prv[hr.src2];
             }
             cur[k] = v;
             if (n == 0) {
@@ -885,9 +573,9 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
           const int kk = it / (ncdc * NHB1), r0 = it - kk * ncdc * NHB1;
           const int cl = r0 / NHB1, k = r0 - cl * NHB1;
           const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
-          const double* Rb = sh.rc[kk] + (t * RB + u) * RB + v;
+          const Real* Rb = sh.rc[kk] + (t * RB + u) * RB + v;
           const int n0 = min(sh.tn[kk][0][cl], TMAX);
-          double h = 0.0;
+          Real h = (Real)0;
           for (int j = 0; j < n0; ++j) h += sh.tc[kk][0][cl][j] * Rb[sh.td[kk][0][cl][j]];
           acc[0] += sh.ya[0][cl][k] * h;
           acc[1] += sh.ya[1][cl][k] * h;
@@ -896,11 +584,11 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
           if (u > 0) acc[4] += sh.y0[cl][herm_index(t, u - 1, v)] * h;
           if (v > 0) acc[5] += sh.y0[cl][herm_index(t, u, v - 1)] * h;
           if (k < NHB0) {
-            const double y0 = sh.y0[cl][k];
+            const Real y0 = sh.y0[cl][k];
 #pragma unroll
             for (int dim = 0; dim < 3; ++dim) {
               const int nn = min(sh.tn[kk][1 + dim][cl], TMAX);
-              double hc = 0.0;
+              Real hc = (Real)0;
               for (int j = 0; j < nn; ++j) hc += sh.tc[kk][1 + dim][cl][j] * Rb[sh.td[kk][1 + dim][cl][j]];
               acc[6 + dim] += y0 * hc;
             }
@@ -909,9 +597,12 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
       }
     }
   }
+  double accd[9];  // the wave sum in FP64 whatever the arithmetic precision
 #pragma unroll
-  for (int i = 0; i < 9; ++i)
-    for (int w = 32; w > 0; w >>= 1) acc[i] += __shfl_xor(acc[i], w);
+  for (int i = 0; i < 9; ++i) {
+    accd[i] = (double)acc[i];
+    for (int w = 32; w > 0; w >>= 1) accd[i] += __shfl_xor(accd[i], w);
+  }
   if (lane == 0) {
     const double f = (args.sp_diag[bp] ? 1.0 : 2.0) * (args.sp_diag[kp] ? 1.0 : 2.0) *
                      (bp == kp ? 1.0 : 2.0) * 0.5;
@@ -919,7 +610,7 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
     const int C = args.sp_atom[2 * kp], D = args.sp_atom[2 * kp + 1];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const double gA = f * acc[k], gP = f * acc[3 + k], gC = f * acc[6 + k];
+      const double gA = f * accd[k], gP = f * accd[3 + k], gC = f * accd[6 + k];
       atomicAdd(&args.grad[3 * A + k], gA);
       atomicAdd(&args.grad[3 * B + k], gP - gA);
       atomicAdd(&args.grad[3 * C + k], gC);
@@ -928,10 +619,11 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
   }
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename Real = double>
 hipError_t grad2e_launch_wave(const Grad2eArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
-  hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD>), grid_1d(a.nquartets, 64), dim3(64), 0, s, a);
+  hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD, Real>), grid_1d(a.nquartets, 64), dim3(64),
+                     0, s, a);
   return hipGetLastError();
 }
 

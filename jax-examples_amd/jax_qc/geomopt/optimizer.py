@@ -36,7 +36,7 @@ class OptResult(NamedTuple):
 
 
 def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12, tau32=None,
-                        tile_f32=None, grad_tau=1e-13, grad_tau_density=0.0):
+                        tile_f32=None, grad_tau=1e-13, grad_tau_density=0.0, grad_tau32=None):
     m = displaced(mol, coords)
     bs = build_basis_set(m, basis_name)
     ints = compute_integrals(m, bs, eri=eri, tau=tau, tau32=tau32)
@@ -50,7 +50,7 @@ def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12,
             ints.ERI.use_compact(tau_f32=tile_f32)
     res = run_scf(m, ints, config)
     g = scf_gradient(m, bs, res, grad_tau, pair_table=getattr(ints.ERI, "pt", None),
-                     tau_density=grad_tau_density)
+                     tau_density=grad_tau_density, tau32=grad_tau32)
     # drop the integral store (tens of GB at 480 functions) before the
     # caller's next evaluation
     res = dataclasses.replace(res, ERI=None)

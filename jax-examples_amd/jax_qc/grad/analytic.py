@@ -24,8 +24,7 @@ from jax_qc.integrals import engine as eng
 
 # Kernel form per (bra class, ket class) of the derivative-ERI contraction:
 # 1 one thread per quartet, 2 one wave per quartet (bra side contracted with
-# G first), 3 one wave per quartet (direct contraction), 0 the library's
-# default.
+# G first), 0 the library's default.
 # Classes absent from the table use the default; a probe may set
 # GRAD_VARIANT_OVERRIDE to force one form for every class that has it.
 GRAD_VARIANTS: dict = {}
@@ -73,10 +72,13 @@ def _densities(mol, res):
 
 
 def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_table=None,
-                        parts=("1e", "2e"), tau_density: float = 0.0) -> torch.Tensor:
+                        parts=("1e", "2e"), tau_density: float = 0.0,
+                        tau32: float | None = None) -> torch.Tensor:
     """sum D d(T+V) - sum W dS + 1/2 sum G dERI on the device, (natom, 3).
     tau: Schwarz threshold of the quartet list; tau_density > 0 also skips
-    quartets with max|G_abcd| Q_ab Q_cd below it (density-weighted)."""
+    quartets with max|G_abcd| Q_ab Q_cd below it (density-weighted); tau32:
+    quartets with Schwarz bound below it sum their primitives in FP32 (the
+    contributions are added in FP64)."""
     dev = nat.device()
     nat.ensure_init(dev)
     pt = pair_table or eng.PairTable(basis, dev)
@@ -104,7 +106,7 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
                      nat.ptr(pt.sp_diag), nat.ptr(ld), len(lst), nat_, nat.ptr(xyz), nat.ptr(zz),
                      pt.ncart, nat.ptr(Djc), nat.ptr(Wc), nat.ptr(grad), s)
     if "2e" in parts:
-        plan = eng.plan_quartets(pt, tau)
+        plan = eng.plan_quartets(pt, tau, tau32)
         Qd = torch.from_numpy(pt.Q).to(dev) if tau_density > 0.0 else None
         for L in plan.launches:
             a = nat.QcGrad2eArgs()
@@ -128,6 +130,9 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
             a.cx = cx
             a.grad = grad.data_ptr()
             a.variant = grad_variant_for(L.cab, L.ccd)
+            a.fp32 = L.fp32
+            if L.fp32:
+                a.pairs32 = pt.pairs32.data_ptr()
             if Qd is not None:
                 a.schwarz = Qd.data_ptr()
                 a.tau_density = tau_density
@@ -136,9 +141,9 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
 
 
 def scf_gradient(mol, basis, res, tau: float = 1e-13, pair_table=None,
-                 tau_density: float = 0.0) -> torch.Tensor:
+                 tau_density: float = 0.0, tau32: float | None = None) -> torch.Tensor:
     """Full nuclear gradient (Eh/Bohr) of a converged RHF or UHF result."""
     Dj, Dk, cx, W = _densities(mol, res)
     g = electronic_gradient(mol, basis, Dj, Dk, cx, W, tau, pair_table,
-                            tau_density=tau_density)
+                            tau_density=tau_density, tau32=tau32)
     return g + torch.from_numpy(nuclear_repulsion_gradient(mol)).to(g.device)

@@ -96,3 +96,31 @@ def test_water4_gradient_all_36_coordinates_vs_fd():
     print("max |analytic - FD| =", np.abs(g - gn).max(), flush=True)
     np.testing.assert_allclose(g, gn, atol=1e-6)
     np.testing.assert_allclose(g.sum(axis=0), 0.0, atol=1e-8)
+
+
+def test_kernel_forms_fp32_split_and_density_screening():
+    """The two derivative-ERI kernel forms (one thread / one wave per
+    quartet) agree to rounding; the FP32 split (Schwarz bound < 1e-4) and
+    density-weighted screening (1e-11) stay within 1e-7 Eh/Bohr of FP64 on
+    (H2O)_2 even-tempered (seed 1)."""
+    import jax_qc
+    from jax_qc.grad import analytic as an
+
+    mol = M.water_cluster((2, 1, 1), seed=1)
+    bs = build_basis_set(mol, "even-tempered")
+    ints = jax_qc.compute_integrals(mol, bs, eri="store", tau=1e-14)
+    res = jax_qc.run_scf(mol, ints, CalcConfig(method="rhf", scf_conv=1e-12))
+    pt = ints.ERI.pt
+    g = {}
+    try:
+        for ov in (1, 2):
+            an.GRAD_VARIANT_OVERRIDE = ov
+            g[ov] = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt).cpu().numpy()
+    finally:
+        an.GRAD_VARIANT_OVERRIDE = None
+    np.testing.assert_allclose(g[1], g[2], atol=1e-11)
+    g32 = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt, tau32=1e-4).cpu().numpy()
+    print("max |fp32 split - fp64| =", np.abs(g32 - g[2]).max(), flush=True)
+    np.testing.assert_allclose(g32, g[2], atol=1e-7)
+    gd = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt, tau_density=1e-11).cpu().numpy()
+    np.testing.assert_allclose(gd, g[2], atol=1e-7)
