@@ -126,6 +126,27 @@ def _fock_rate(store, M, D, H, n, steps, warmup):
     return steps / dt, dt, avg_ms
 
 
+def read_ceiling(nbytes: int) -> float:
+    """GB/s of torch's sum reduction over a buffer of the same byte count:
+    what a plain read stream reaches on this device (a practical ceiling
+    beside the 8 TB/s specification)."""
+    import torch
+
+    buf = torch.ones(max(1, nbytes // 8), dtype=torch.float64, device="cuda")
+    buf.sum()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        buf.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    del buf
+    torch.cuda.empty_cache()
+    return nbytes / (ms * 1e-3) / 1e9
+
+
 def eri_report(store):
     """Per-class ERI kernel time (HIP events around a re-run of each class
     launch into a scratch supermatrix) and algorithmic FLOP rate, by
@@ -472,6 +493,7 @@ def main():
         pick = args.mode
     P = results[pick]
     achieved = P["bytes"] / (P["avg_ms"] * 1e-3) / 1e9
+    ceiling = read_ceiling(P["bytes"])
     cfg = {"workload": f"(H2O)_{int(np.prod(grid))} even-tempered RHF Fock build",
            "grid": list(grid), "seed": 0, "n_basis": n, "n_shells": len(bs.shells),
            "schwarz_tau": 1e-12, "path": pick, "dE_mixed_vs_fp64": dE,
@@ -505,7 +527,9 @@ def main():
                                  else None),
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated",
                      "kernel": "symv_tiles_kernel<1,%s>" % ("true" if pick == "mixed" else "false"),
-                     "bytes_per_launch": P["bytes"], "avg_launch_ms": round(P["avg_ms"], 5)},
+                     "bytes_per_launch": P["bytes"], "avg_launch_ms": round(P["avg_ms"], 5),
+                     "read_ceiling_gbs": round(ceiling, 1),
+                     "frac_of_read_ceiling": round(achieved / ceiling, 4)},
     }
     if not (args.no_cpu or args.profile_steps):
         out["cpu_baseline"] = cpu_baseline(n)

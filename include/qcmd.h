@@ -74,7 +74,10 @@ int qc_eri(int cab, int ccd, int fp32, const qc_eri_args* args, void* stream);
 
 /* One-electron S, T, V (which = 0, 1, 2) for all shell pairs of pair class
  * cab, written (both triangles) into the dense n x n matrix M.
- * Replaces overlap.py:67-85, kinetic.py:42-59, nuclear.py:62-81. */
+ * Replaces overlap.py:67-85, kinetic.py:42-59, nuclear.py:62-81.
+ * which = 3, 4, 5: exact dipole integrals <mu| x_k - O_k |nu> about the
+ * origin O = atom_xyz[0..2] (natom, atom_z unused); they replace the
+ * approximate ones of jax_qc/properties/dipole.py:26-89. */
 int qc_one_electron(int which, int cab, const double* pairs64, const int* sp_prim_off,
                     const int* sp_nprim, const int* sp_ao, const int* pair_list, int npairs,
                     int natom, const double* atom_xyz, const double* atom_z, int64_t nbf,

@@ -6,6 +6,10 @@
 //   T_ij^x = 1/2 [4ab S_{i+1,j+1} + ij S_{i-1,j-1} - 2aj S_{i+1,j-1} - 2bi S_{i-1,j+1}]
 //   T = K (T^x S^y S^z + S^x T^y S^z + S^x S^y T^z)
 //   V = -sum_C Z_C (2 pi/p) K sum_tuv E^x_t E^y_u E^z_v R_tuv(p, P-C)
+//   dipole (which = 3, 4, 5 for x, y, z about the origin O):
+//   M_ij^x = sqrt(pi/p) (E_ij^1 + (P_x - O_x) E_ij^0),  D_x = K M^x S^y S^z
+// (the first Hermite moment: int x_O Lambda_t = sqrt(pi/p) [t = 1] +
+// X_PO sqrt(pi/p) [t = 0]).
 // K already carries the contraction coefficients.  The Cartesian block goes to
 // the spherical basis (overlap.py:58-64) and is written to both triangles.
 #include "qcmd_device.h"
@@ -95,6 +99,27 @@ __global__ __launch_bounds__(128) void one_electron_kernel(
           Sz[i][j] = f * Ez[i][j][0];
         }
       double Tx[LA + 1][LB + 1], Ty[LA + 1][LB + 1], Tz[LA + 1][LB + 1];
+      // dipole: first moment about the origin (axyz) in the field direction
+      // (three branches keep every table index compile-time)
+      if (which == 3) {
+        const double xo = P[PP_PX] - axyz[0];
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+          for (int j = 0; j <= LB; ++j) Sx[i][j] = f * (Ex[i][j][1] + xo * Ex[i][j][0]);
+      } else if (which == 4) {
+        const double yo = P[PP_PX + 1] - axyz[1];
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+          for (int j = 0; j <= LB; ++j) Sy[i][j] = f * (Ey[i][j][1] + yo * Ey[i][j][0]);
+      } else if (which == 5) {
+        const double zo = P[PP_PX + 2] - axyz[2];
+#pragma unroll
+        for (int i = 0; i <= LA; ++i)
+#pragma unroll
+          for (int j = 0; j <= LB; ++j) Sz[i][j] = f * (Ez[i][j][1] + zo * Ez[i][j][0]);
+      }
       if (which == 1) {
 #pragma unroll
         for (int i = 0; i <= LA; ++i)
@@ -129,7 +154,7 @@ __global__ __launch_bounds__(128) void one_electron_kernel(
           const int ax = CX[LA][a], ay = CY[LA][a], az_ = CZ[LA][a];
           const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
           double v;
-          if (which == 0)
+          if (which == 0 || which >= 3)
             v = Sx[ax][bx] * Sy[ay][by] * Sz[az_][bz];
           else
             v = Tx[ax][bx] * Sy[ay][by] * Sz[az_][bz] + Sx[ax][bx] * Ty[ay][by] * Sz[az_][bz] +
@@ -165,7 +190,8 @@ static int one_electron(int which, int cab, const double* pairs64, const int* sp
                         const int* sp_nprim, const int* sp_ao, const int* pair_list, int npairs,
                         int natom, const double* atom_xyz, const double* atom_z, int64_t nbf,
                         double* M, const int* pair_atoms, const int64_t* pair_base, void* stream) {
-  if (which < 0 || which > 2) return set_error("qc_one_electron: which=%d", which);
+  if (which < 0 || which > 5) return set_error("qc_one_electron: which=%d", which);
+  if (which >= 3 && !atom_xyz) return set_error("qc_one_electron: dipole needs an origin");
   if (npairs <= 0) return 0;
   const double* boys = boys_table64();
   if (!boys) return set_error("qc_one_electron: qc_init not called on the current device");

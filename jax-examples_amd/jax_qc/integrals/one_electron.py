@@ -11,13 +11,17 @@ from jax_qc import _native as nat
 from jax_qc.integrals import engine as eng
 
 
-def _one(basis, which: int, mol=None, pair_table=None) -> torch.Tensor:
+def _one(basis, which: int, mol=None, pair_table=None, origin=None) -> torch.Tensor:
     dev = nat.device()
     nat.ensure_init(dev)
     pt = pair_table or eng.PairTable(basis, dev, fp32=False)
     n = pt.nbf
     M = torch.zeros((n, n), dtype=torch.float64, device=dev)
-    if mol is not None:
+    if origin is not None:  # dipole integrals: the origin goes where the nuclei would
+        xyz = torch.as_tensor(np.asarray(origin, dtype=np.float64).reshape(3), device=dev)
+        zz = None
+        nat_atoms = 0
+    elif mol is not None:
         xyz = torch.as_tensor(np.ascontiguousarray(mol.coords, dtype=np.float64), device=dev)
         zz = torch.as_tensor(np.asarray(mol.atomic_numbers, dtype=np.float64), device=dev)
         nat_atoms = len(zz)
@@ -57,3 +61,10 @@ def nuclear_repulsion_energy(mol) -> float:
         i, j = [int(x[np.argmin(d[iu])]) for x in iu]
         raise ValueError(f"Atoms {i} and {j} are coincident; cannot compute E_nuc.")
     return float(np.sum(np.outer(Z, Z)[iu] / d[iu]))
+
+
+def compute_dipole_integrals(basis, origin=(0.0, 0.0, 0.0), pair_table=None) -> torch.Tensor:
+    """(3, n, n) exact dipole integrals <mu| r_k - O_k |nu> (the reference
+    approximates them, jax_qc/properties/dipole.py:26-89)."""
+    return torch.stack([_one(basis, 3 + k, pair_table=pair_table, origin=origin)
+                        for k in range(3)])
