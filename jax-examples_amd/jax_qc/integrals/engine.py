@@ -80,6 +80,39 @@ def shell_pair_records(sa, a: int, b: int, prim_cut: float = 0.0) -> np.ndarray:
     return rec
 
 
+def all_pair_records(sa, pa, pb, prim_cut: float = 0.0):
+    """``shell_pair_records`` for many shell pairs (pa[k], pb[k]) at once:
+    returns the concatenated records (pair by pair, primitives of pa outer)
+    and the per-pair record counts."""
+    ex, co, ce, off, npr = (sa[k] for k in ("exps", "coefs", "centers", "prim_off", "nprim"))
+    na, nb = npr[pa].astype(np.int64), npr[pb].astype(np.int64)
+    cnt = na * nb
+    pid = np.repeat(np.arange(len(pa)), cnt)
+    start = np.zeros(len(pa), dtype=np.int64)
+    start[1:] = np.cumsum(cnt)[:-1]
+    local = np.arange(int(cnt.sum()), dtype=np.int64) - start[pid]
+    ka = off[pa][pid] + local // nb[pid]
+    kb = off[pb][pid] + local % nb[pid]
+    al, be = ex[ka], ex[kb]
+    A, B = ce[pa][pid], ce[pb][pid]
+    p = al + be
+    P = (al[:, None] * A + be[:, None] * B) / p[:, None]
+    AB2 = np.sum((A - B) ** 2, axis=1)
+    K = co[ka] * co[kb] * np.exp(-al * be / p * AB2)
+    rec = np.empty((len(p), 12))
+    rec[:, 0] = p
+    rec[:, 1:4] = P
+    rec[:, 4:7] = P - A
+    rec[:, 7:10] = P - B
+    rec[:, 10] = K
+    rec[:, 11] = al
+    if prim_cut > 0.0:
+        keep = np.abs(K) >= prim_cut
+        rec = rec[keep]
+        cnt = np.bincount(pid[keep], minlength=len(pa)).astype(np.int64)
+    return rec, cnt
+
+
 class PairTable:
     """Shell-pair and primitive-pair tables for one basis on one device."""
 
@@ -101,30 +134,25 @@ class PairTable:
         self.ncart = int(aoc[-1])
         self.cart_off = aoc
         atom = sa["atom"]
-        pairs, sp_off, sp_n, sp_ao, sp_cls, sp_sh = [], [], [], [], [], []
-        npp = 0
-        for i in range(self.nshell):
-            for j in range(i + 1):
-                a, b = (i, j) if L[i] >= L[j] else (j, i)
-                rec = shell_pair_records(sa, a, b, prim_cut)
-                pairs.append(rec)
-                sp_off.append(npp)
-                sp_n.append(len(rec))
-                npp += len(rec)
-                sp_ao.append((ao[a], ao[b]))
-                sp_cls.append(pair_class(int(L[a]), int(L[b])))
-                sp_sh.append((a, b))
-        rec = np.concatenate(pairs) if pairs else np.zeros((0, 12))
-        self.npairs = len(sp_n)
-        self.sp_class = np.array(sp_cls, dtype=np.int32)
-        self.sp_shells = np.array(sp_sh, dtype=np.int32).reshape(-1, 2)
-        self.sp_ao_host = np.array(sp_ao, dtype=np.int32).reshape(-1, 2)
+        # shell pairs i >= j in (i, j) order, oriented larger l first
+        ii, jj = np.tril_indices(self.nshell)
+        swap = L[ii] < L[jj]
+        pa = np.where(swap, jj, ii).astype(np.int64)
+        pb = np.where(swap, ii, jj).astype(np.int64)
+        rec, sp_n = all_pair_records(sa, pa, pb, prim_cut)
+        sp_off = np.zeros(len(pa), dtype=np.int64)
+        sp_off[1:] = np.cumsum(sp_n)[:-1]
+        npp = int(sp_n.sum())
+        self.npairs = len(pa)
+        self.sp_class = (L[pa] * (L[pa] + 1) // 2 + L[pb]).astype(np.int32)
+        self.sp_shells = np.stack([pa, pb], 1).astype(np.int32)
+        self.sp_ao_host = np.stack([ao[pa], ao[pb]], 1).astype(np.int32)
         self.nprim_pairs = npp
         d = self.dev
         self.pairs64 = torch.from_numpy(np.ascontiguousarray(rec)).to(d)
         self.pairs32 = self.pairs64.float() if fp32 else None
-        self.sp_prim_off = torch.tensor(sp_off, dtype=torch.int32, device=d)
-        self.sp_nprim = torch.tensor(sp_n, dtype=torch.int32, device=d)
+        self.sp_prim_off = torch.from_numpy(sp_off.astype(np.int32)).to(d)
+        self.sp_nprim = torch.from_numpy(np.asarray(sp_n, dtype=np.int32)).to(d)
         self.sp_ao = torch.from_numpy(self.sp_ao_host.copy()).to(d)
         a_, b_ = self.sp_shells[:, 0], self.sp_shells[:, 1]
         self.sp_aoc = torch.from_numpy(

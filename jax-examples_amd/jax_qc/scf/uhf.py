@@ -22,7 +22,15 @@ from jax_qc.scf.energy import electronic_energy_uhf
 from jax_qc.scf.fock import build_fock_uhf
 from jax_qc.scf.guess import core_guess_uhf
 from jax_qc.scf.orthogonalize import symmetric_orthogonalization
-from jax_qc.scf.rhf import _diagonalize_fock
+
+
+def _diagonalize_pair(Fa, Fb, X):
+    """Both spins' Roothaan problems in one batched eigensolver call (the
+    solver's cost at a few hundred functions is latency, not arithmetic, so
+    two matrices cost about what one does)."""
+    Xt = X.T.contiguous()
+    e, Cp = torch.linalg.eigh(Xt @ torch.stack([Fa, Fb]) @ X)
+    return (e[0], X @ Cp[0]), (e[1], X @ Cp[1])
 
 
 @dataclass
@@ -50,8 +58,7 @@ def scf_step_uhf(state, H, ERI, S, X, n_alpha, n_beta, hist, damping, timer=None
         Fa, Fb, hist, ea, eb = diis_extrapolate_uhf(hist, Fa, Fb, state.density_alpha,
                                                     state.density_beta, S, X)
     with optional_stage(timer, "diag"):
-        oa, Ca = _diagonalize_fock(Fa, X)
-        ob, Cb = _diagonalize_fock(Fb, X)
+        (oa, Ca), (ob, Cb) = _diagonalize_pair(Fa, Fb, X)
     with optional_stage(timer, "density"):
         Da, Db = density_uhf(Ca, Cb, n_alpha, n_beta)
     with optional_stage(timer, "energy"):
@@ -71,8 +78,7 @@ def run_uhf(mol: Molecule, ints: IntegralSet, config: CalcConfig = CalcConfig(),
         with optional_stage(timer, "initial_guess"):
             Da, Db, _ = core_guess_uhf(H, X, na, nb, ints.ERI)
             Fa, Fb = build_fock_uhf(H, Da, Db, ints.ERI)
-            oa, Ca = _diagonalize_fock(Fa, X)
-            ob, Cb = _diagonalize_fock(Fb, X)
+            (oa, Ca), (ob, Cb) = _diagonalize_pair(Fa, Fb, X)
             Da, Db = density_uhf(Ca, Cb, na, nb)
             E0 = float(electronic_energy_uhf(Da, Db, H, Fa, Fb))
         state = UHFState(Da, Db, Fa, Fb, Ca, Cb, oa, ob, E0, 0)
