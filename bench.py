@@ -470,7 +470,8 @@ def main():
         nbytes = Mx.bytes if isinstance(Mx, CompactSupermatrix) else eng.sm_numel(n) * 8
         r = dict(rate=rate, dt=dt, avg_ms=avg_ms, bytes=nbytes, t_ints=t_ints, t_scf=t_scf,
                  eri_s=store.eri_seconds, n64=store.plan.n64, n32=store.plan.n32,
-                 nscr=store.plan.n_screened)
+                 nscr=store.plan.n_screened,
+                 eri_quartets_per_s=(store.plan.n64 + store.plan.n32) / store.eri_seconds)
         if res is not None:
             r.update(energy=res.energy, iters=res.n_iterations, converged=res.converged,
                      t_grad=t_grad, energy_grad_wall_s=t_ints + t_scf + t_grad)

@@ -124,11 +124,12 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
   for (int ik = 0; ik < npk; ++ik) {
     const double* Q = pp + (int64_t)(pk0 + ik) * PP_STRIDE;
     const double q = Q[PP_P], ga2 = 2.0 * Q[PP_ALPHA];
-    double Ek[3][LC + 2][LD + 2][LCD + 3];
+    // ket tables up to (lc+1, ld): the C derivative only (D by invariance)
+    double Ek[3][LC + 2][LD + 1][LCD + 2];
     double dEk[3][LC + 1][LD + 1][LCD + 2];
 #pragma unroll
     for (int dim = 0; dim < 3; ++dim) {
-      hermite_e<LC + 1, LD + 1, double>(Ek[dim], Q[PP_PAX + dim], Q[PP_PBX + dim], 0.5 / q);
+      hermite_e<LC + 1, LD, double>(Ek[dim], Q[PP_PAX + dim], Q[PP_PBX + dim], 0.5 / q);
 #pragma unroll
       for (int i = 0; i <= LC; ++i)
 #pragma unroll
@@ -142,12 +143,14 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
     }
     for (int ib = 0; ib < npb; ++ib) {
       const double* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
-      const double p = P[PP_P], al2 = 2.0 * P[PP_ALPHA], be2 = 2.0 * (p - P[PP_ALPHA]);
-      double Eb[3][LA + 2][LB + 2][LAB + 3];
-      double dEa[3][LA + 1][LB + 1][LAB + 2], dEb[3][LA + 1][LB + 1][LAB + 2];
+      const double p = P[PP_P], al2 = 2.0 * P[PP_ALPHA];
+      // bra tables up to (la+1, lb) and the A derivative only: B follows from
+      // the Hermite shift d/dP R_tuv = R_{t+1,u,v} (dB = dP - dA)
+      double Eb[3][LA + 2][LB + 1][LAB + 2];
+      double dEa[3][LA + 1][LB + 1][LAB + 2];
 #pragma unroll
       for (int dim = 0; dim < 3; ++dim) {
-        hermite_e<LA + 1, LB + 1, double>(Eb[dim], P[PP_PAX + dim], P[PP_PBX + dim], 0.5 / p);
+        hermite_e<LA + 1, LB, double>(Eb[dim], P[PP_PAX + dim], P[PP_PBX + dim], 0.5 / p);
 #pragma unroll
         for (int i = 0; i <= LA; ++i)
 #pragma unroll
@@ -156,10 +159,7 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
             for (int t = 0; t <= LAB + 1; ++t) {
               double va = al2 * Eb[dim][i + 1][j][t];
               if (i > 0) va -= (double)i * Eb[dim][i > 0 ? i - 1 : 0][j][t];
-              double vb = be2 * Eb[dim][i][j + 1][t];
-              if (j > 0) vb -= (double)j * Eb[dim][i][j > 0 ? j - 1 : 0][t];
               dEa[dim][i][j][t] = va;
-              dEb[dim][i][j][t] = vb;
             }
       }
       const double pqm = p * q, ps = p + q, alpha = pqm / ps;
@@ -231,20 +231,15 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
                     if (over > 1) continue;
                     const double e0x = Eb[0][ax][bx][t], e0y = Eb[1][ay][by][u], e0z = Eb[2][az][bz][v];
                     const double hv = h0[herm_index(t, u, v)];
-                    if (u <= ay + by && v <= az + bz) {
-                      s[0] += dEa[0][ax][bx][t] * e0y * e0z * hv;
-                      s[3] += dEb[0][ax][bx][t] * e0y * e0z * hv;
-                    }
-                    if (t <= ax + bx && v <= az + bz) {
-                      s[1] += e0x * dEa[1][ay][by][u] * e0z * hv;
-                      s[4] += e0x * dEb[1][ay][by][u] * e0z * hv;
-                    }
-                    if (t <= ax + bx && u <= ay + by) {
-                      s[2] += e0x * e0y * dEa[2][az][bz][v] * hv;
-                      s[5] += e0x * e0y * dEb[2][az][bz][v] * hv;
-                    }
+                    if (u <= ay + by && v <= az + bz) s[0] += dEa[0][ax][bx][t] * e0y * e0z * hv;
+                    if (t <= ax + bx && v <= az + bz) s[1] += e0x * dEa[1][ay][by][u] * e0z * hv;
+                    if (t <= ax + bx && u <= ay + by) s[2] += e0x * e0y * dEa[2][az][bz][v] * hv;
                     if (over == 0) {
                       const double e0 = e0x * e0y * e0z;
+                      // P = A + B: the bra distribution moved along x, y, z
+                      s[3] += e0 * h0[herm_index(t + 1, u, v)];
+                      s[4] += e0 * h0[herm_index(t, u + 1, v)];
+                      s[5] += e0 * h0[herm_index(t, u, v + 1)];
                       s[6] += e0 * hC[0][herm_index(t, u, v)];
                       s[7] += e0 * hC[1][herm_index(t, u, v)];
                       s[8] += e0 * hC[2][herm_index(t, u, v)];
@@ -263,11 +258,11 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
   const int C = args.sp_atom[2 * kp], D = args.sp_atom[2 * kp + 1];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double gA = f * acc[k], gB = f * acc[3 + k], gC = f * acc[6 + k];
+    const double gA = f * acc[k], gP = f * acc[3 + k], gC = f * acc[6 + k];
     atomicAdd(&args.grad[3 * A + k], gA);
-    atomicAdd(&args.grad[3 * B + k], gB);
+    atomicAdd(&args.grad[3 * B + k], gP - gA);
     atomicAdd(&args.grad[3 * C + k], gC);
-    atomicAdd(&args.grad[3 * D + k], -(gA + gB + gC));
+    atomicAdd(&args.grad[3 * D + k], -(gP + gC));
   }
 }
 
