@@ -408,7 +408,8 @@ class BatchedRHF:
         E = self._energy(D, F0)
         last = E.cpu().numpy()
         focks, errs = [], []
-        gram = np.zeros((self.nmol, 0, 0))  # <e_i, e_j> per molecule, in step with the ring
+        # <e_i, e_j> per molecule, kept on the device in step with the ring
+        gram = torch.zeros(self.nmol, 0, 0, dtype=torch.float64, device=self.dev)
         done = np.zeros(self.nmol, dtype=bool)
         n_it = np.zeros(self.nmol, dtype=np.int64)
         E_final = last.copy()
@@ -425,30 +426,27 @@ class BatchedRHF:
                 gram = gram[:, 1:, 1:]
             focks.append(F)
             errs.append(err)
-            row = torch.stack([(x * err).sum(dim=(1, 2)) for x in errs], 1).cpu().numpy()
+            row = torch.stack([(x * err).sum(dim=(1, 2)) for x in errs], 1)
             m = len(errs)
-            G = np.empty((self.nmol, m, m))
+            G = torch.empty(self.nmol, m, m, dtype=torch.float64, device=self.dev)
             G[:, :-1, :-1] = gram
             G[:, -1, :] = row
             G[:, :, -1] = row
             gram = G
-            if len(focks) >= 2:
-                coef = np.zeros((self.nmol, len(focks)))
-                m = len(focks)
-                for i in range(self.nmol):
-                    B = np.empty((m + 1, m + 1))
-                    B[:m, :m] = G[i]
-                    B[-1, :-1] = -1.0
-                    B[:-1, -1] = -1.0
-                    B[-1, -1] = 0.0
-                    rhs = np.zeros(m + 1)
-                    rhs[-1] = -1.0
-                    try:
-                        coef[i] = np.linalg.solve(B, rhs)[:-1]
-                    except np.linalg.LinAlgError:
-                        coef[i] = 0.0
-                        coef[i, -1] = 1.0
-                ct = torch.from_numpy(coef).to(self.dev)
+            if m >= 2:
+                # Pulay system per molecule, all molecules in one batched solve;
+                # an exactly singular system keeps the un-extrapolated Fock
+                # (jax_qc/scf/diis.py:97-100)
+                B = torch.zeros(self.nmol, m + 1, m + 1, dtype=torch.float64, device=self.dev)
+                B[:, :m, :m] = G
+                B[:, m, :m] = -1.0
+                B[:, :m, m] = -1.0
+                rhs = torch.zeros(self.nmol, m + 1, dtype=torch.float64, device=self.dev)
+                rhs[:, m] = -1.0
+                sol, info = torch.linalg.solve_ex(B, rhs)
+                fallback = torch.zeros(self.nmol, m, dtype=torch.float64, device=self.dev)
+                fallback[:, -1] = 1.0
+                ct = torch.where((info == 0)[:, None], sol[:, :m], fallback)
                 Fx = torch.einsum("ni,nijk->njk", ct, torch.stack(focks, 1))
             else:
                 Fx = F
