@@ -21,10 +21,11 @@ import torch
 from jax_qc.integrals import engine as eng
 
 # (cab, ccd) -> variant, pair classes ss=0 ps=1 pp=2 ds=3 dp=4 dd=5.
-# Measured (FP64, per launch): per-thread wins for (dp|pp) 15.7 vs 141 ms,
-# (dp|ds) 4.5 vs 62, (dd|ps) 3.2 vs 29, (dd|ds) 6.2 vs 13; the wave kernel
-# wins for (dp|dp) 24 vs 46, (dd|pp) 31 vs 69, (dd|dp) 35 vs 124,
-# (dd|dd) 13 vs 322.  Lower classes: per-thread.
+# Measured on MI355X, (H2O)_8 even-tempered, FP64, per launch (one thread vs
+# one wave per quartet): per-thread wins for (dp|pp) 14.9 vs 124 ms, (dp|ds)
+# 4.3 vs 86, (dd|ps) 3.1 vs 22, (dd|ds) 6.3 vs 16.7; the wave kernel wins for
+# (dp|dp) 18.8 vs 55, (dd|pp) 10.3 vs 68, (dd|dp) 11.4 vs 119, (dd|dd) 3.7
+# vs 328.  Lower classes: per-thread (by 5-40x).
 DEFAULT_VARIANTS: Dict[Tuple[int, int], int] = {
     (4, 2): 1, (4, 3): 1, (5, 1): 1, (5, 3): 1,
     (4, 4): 2, (5, 2): 2, (5, 4): 2, (5, 5): 2,
