@@ -43,7 +43,7 @@ FLAGS = [
     # no packed FP32 instructions (v_pk_*_f32) anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
-]
+] + os.environ.get("QCMD_EXTRA_FLAGS", "").split()  # e.g. -DQC_EW_KC_MIN=2 for tuning sweeps
 
 
 def _generate():

@@ -95,8 +95,14 @@ __host__ __device__ constexpr int pair_terms(int la, int lb) {
   return m;
 }
 
-constexpr int EW_BUDGET = 6144;     // 8-byte words of LDS per wave
-constexpr int EW_KC_BUDGET = 2560;  // batching ket primitives stops here (occupancy)
+#ifndef QC_EW_KC_BUDGET
+#define QC_EW_KC_BUDGET 2560
+#endif
+#ifndef QC_EW_KC_MIN
+#define QC_EW_KC_MIN 7
+#endif
+constexpr int EW_BUDGET = 6144;                // 8-byte words of LDS per wave
+constexpr int EW_KC_BUDGET = QC_EW_KC_BUDGET;  // batching ket primitives stops here (occupancy)
 
 // One 64-lane wave per contracted quartet.  Per bra primitive pair: the bra
 // E tables and, per bra Cartesian pair ab, the list of its nonzero
@@ -129,7 +135,7 @@ struct EriWave {
     while (kc > 1 && lds(kc) > EW_KC_BUDGET) --kc;
     return kc;
   }
-  static constexpr int KC = pick_kc() >= 7 ? pick_kc() : 1;
+  static constexpr int KC = pick_kc() >= QC_EW_KC_MIN ? pick_kc() : 1;
   struct PerKet {
     Real ek[KC][3][EKS];
     Real r[2][KC][NHR];
