@@ -112,6 +112,9 @@ constexpr int EW_KC_BUDGET = QC_EW_KC_BUDGET;  // batching ket primitives stops 
 // E^{cd}_{t'u'v'}, box offset) terms, then h_cd[tuv] = sum_terms c R[tuv + t'u'v']
 // (items (cd, tuv) over all lanes), then every lane adds to the Cartesian
 // block elements it owns: V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv].
+// Both term lists are padded with zero terms to the class's largest count, so
+// the term loops have compile-time trip counts and issue all their LDS loads
+// back to back (a wave is latency-bound at one or two waves per SIMD).
 template <int LA, int LB, int LC, int LD, typename Real>
 struct EriWave {
   static constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
@@ -125,9 +128,9 @@ struct EriWave {
   static constexpr int IPL = (NAB * NCD + 63) / 64;  // block elements per lane
   static constexpr int WR = sizeof(Real) == 8 ? 2 : 1;  // Real per 8-byte word, inverse
   // 8-byte words: fixed, per ket primitive of a batch
-  static constexpr int FIX = (3 * EBS + NAB * TB) / (3 - WR) + (NAB * TB + NAB) / 2 + 4;
+  static constexpr int FIX = (3 * EBS + NAB * TB) / (3 - WR) + (NAB * TB) / 2 + 4;
   static constexpr int KPER = (3 * EKS + 2 * NHR + RB * RB * RB + (L + 1) + 4 + NCD * TK +
-                               NCD * NHB) / (3 - WR) + (NCD * TK + NCD) / 2 + 2;
+                               NCD * NHB) / (3 - WR) + (NCD * TK) / 2 + 2;
   static constexpr int BLK = NAB * NCD / (3 - WR) + 1;
   static constexpr int lds(int kc) { return FIX + (kc * KPER > BLK ? kc * KPER : BLK); }
   static constexpr int pick_kc() {
@@ -142,16 +145,14 @@ struct EriWave {
     Real rc[KC][RB * RB * RB];
     Real f[KC][L + 1];
     Real misc[KC][4];  // X, Y, Z, pref
-    Real kc[KC][NCD][TK];
+    Real kc[KC][NCD][TK];  // padded to TK terms with zero coefficients
     int kd[KC][NCD][TK];
-    int kn[KC][NCD];
     Real hb[KC][NCD][NHB];
   };
   struct Shared {
     Real eb[3][EBS];
-    Real bc[NAB][TB];
+    Real bc[NAB][TB];  // padded to TB terms with zero coefficients
     int bi[NAB][TB];
-    int bn[NAB];
     union {
       PerKet k;
       Real blk[NAB * NCD];  // the contracted Cartesian block, after the primitive loops
@@ -215,7 +216,10 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
           }
         }
       }
-      sh.bn[ab] = n;
+      for (; n < TB; ++n) {
+        sh.bc[ab][n] = (Real)0;
+        sh.bi[ab][n] = 0;
+      }
     }
     for (int ik0 = 0; ik0 < npk; ik0 += KC) {
       const int nk = min(KC, npk - ik0);
@@ -273,7 +277,10 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
             }
           }
         }
-        K.kn[kk][cd] = n;
+        for (; n < TK; ++n) {
+          K.kc[kk][cd][n] = (Real)0;
+          K.kd[kk][cd][n] = 0;
+        }
       }
       // R^0_tuv (t+u+v <= L) per ket primitive of the batch, levels n = L
       // .. 0; level 0 also into the RB^3 box
@@ -306,9 +313,10 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
         const int cd = r0 / NHB, k = r0 - cd * NHB;
         const int tuv = kHermDecode<LAB>.tuv[k];
         const Real* Rb = K.rc[kk] + ((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16);
-        const int nn = min(K.kn[kk][cd], TK);
+        // fixed trip count: every offset load issues before the first R load
         Real h = (Real)0;
-        for (int j = 0; j < nn; ++j) h += K.kc[kk][cd][j] * Rb[K.kd[kk][cd][j]];
+#pragma unroll
+        for (int j = 0; j < TK; ++j) h += K.kc[kk][cd][j] * Rb[K.kd[kk][cd][j]];
         K.hb[kk][cd][k] = h;
       }
       __syncthreads();
@@ -318,10 +326,10 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
         const int it = lane + 64 * j;
         if (it < NAB * NCD) {
           const int ab = it / NCD, cd = it - ab * NCD;
-          const int nb = min(sh.bn[ab], TB);
           Real s = (Real)0;
           for (int kk = 0; kk < nk; ++kk)
-            for (int m = 0; m < nb; ++m) s += sh.bc[ab][m] * K.hb[kk][cd][sh.bi[ab][m]];
+#pragma unroll
+            for (int m = 0; m < TB; ++m) s += sh.bc[ab][m] * K.hb[kk][cd][sh.bi[ab][m]];
           acc[j] += s;
         }
       }

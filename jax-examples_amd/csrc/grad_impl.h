@@ -326,13 +326,14 @@ struct GradWave {
   // most ket terms (t', u', v') of one Cartesian pair cd: list 0 for h, list
   // 1 + x for hC_x (one more term along x)
   static constexpr int TMAX = tmax_terms(LC, LD);
+  static constexpr int T0 = pair_terms(LC, LD);  // list 0 has no extra order
   // LDS bytes: fixed part, per ket pair of a chunk, per ket primitive of a
   // batch, per (ket pair, ket primitive)
   static constexpr int RS = sizeof(Real);
   static constexpr int FIX = (NAB * NCD + 3 * EBS) * RS + 64;
   static constexpr int YPER = (NHB0 + 3 * NHB1) * RS;
   static constexpr int KPER = (3 * EKS + 2 * NHR + RB * RB * RB + (L1 + 1) + 4) * RS;
-  static constexpr int TPER = 4 * TMAX * (RS + 4) + 16;
+  static constexpr int TPER = 4 * TMAX * (RS + 4);
   static constexpr int lds(int nc, int kc) { return FIX + nc * YPER + kc * (KPER + nc * TPER); }
   // as many ket primitives per batch (at most GMAXP) as keep the whole ket
   // side within GW_KC_BUDGET (occupancy); ket pairs are then chunked to fit
@@ -362,9 +363,11 @@ struct GradWave {
     Real rc[KC][RB * RB * RB];
     Real f[KC][L1 + 1];
     Real misc[KC][4];           // X, Y, Z, pref
-    Real tc[KC][4][NCDC][TMAX];  // term coefficients (sign, pref and derivative factors folded in)
-    int td[KC][4][NCDC][TMAX];     // term offsets in the R box
-    int tn[KC][4][NCDC];           // term counts
+    // term coefficients (sign, pref and derivative factors folded in) and
+    // offsets in the R box; list 0 padded with zero terms to T0, lists 1-3 to
+    // TMAX, so that the term loops have compile-time trip counts
+    Real tc[KC][4][NCDC][TMAX];
+    int td[KC][4][NCDC][TMAX];
   };
 };
 
@@ -373,7 +376,7 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
   using W = GradWave<LA, LB, LC, LD, Real>;
   constexpr int LAB = W::LAB, LCD = W::LCD, L1 = W::L1, NB = W::NB, ND = W::ND;
   constexpr int NAB = W::NAB, NCD = W::NCD, NHB0 = W::NHB0, NHB1 = W::NHB1, NHR = W::NHR;
-  constexpr int NCDC = W::NCDC, KC = W::KC, RB = W::RB, TMAX = W::TMAX;
+  constexpr int NCDC = W::NCDC, KC = W::KC, RB = W::RB, TMAX = W::TMAX, T0 = W::T0;
   __shared__ typename W::Shared sh;
   const HermTable<L1>& HT = kHermTable<L1>;
   const HermDecode<LAB + 1>& HD = kHermDecode<LAB + 1>;
@@ -536,7 +539,10 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
               }
             }
           }
-          sh.tn[kk][l][cl] = n;
+          for (const int nmax = l == 0 ? T0 : TMAX; n < nmax; ++n) {
+            sh.tc[kk][l][cl][n] = (Real)0;
+            sh.td[kk][l][cl][n] = 0;
+          }
         }
         // R^0_tuv (t+u+v <= L+1) of every primitive quartet of the batch,
         // levels n = L+1 .. 0, ping-pong buffers; level 0 also goes into the
@@ -569,9 +575,9 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
           const int cl = r0 / NHB1, k = r0 - cl * NHB1;
           const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
           const Real* Rb = sh.rc[kk] + (t * RB + u) * RB + v;
-          const int n0 = min(sh.tn[kk][0][cl], TMAX);
           Real h = (Real)0;
-          for (int j = 0; j < n0; ++j) h += sh.tc[kk][0][cl][j] * Rb[sh.td[kk][0][cl][j]];
+#pragma unroll
+          for (int j = 0; j < T0; ++j) h += sh.tc[kk][0][cl][j] * Rb[sh.td[kk][0][cl][j]];
           acc[0] += sh.ya[0][cl][k] * h;
           acc[1] += sh.ya[1][cl][k] * h;
           acc[2] += sh.ya[2][cl][k] * h;
@@ -582,9 +588,9 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
             const Real y0 = sh.y0[cl][k];
 #pragma unroll
             for (int dim = 0; dim < 3; ++dim) {
-              const int nn = min(sh.tn[kk][1 + dim][cl], TMAX);
               Real hc = (Real)0;
-              for (int j = 0; j < nn; ++j) hc += sh.tc[kk][1 + dim][cl][j] * Rb[sh.td[kk][1 + dim][cl][j]];
+#pragma unroll
+              for (int j = 0; j < TMAX; ++j) hc += sh.tc[kk][1 + dim][cl][j] * Rb[sh.td[kk][1 + dim][cl][j]];
               acc[6 + dim] += y0 * hc;
             }
           }
