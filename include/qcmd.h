@@ -211,6 +211,9 @@ typedef struct {
   int fp32;              /* 1: primitive sums in FP32 (wave kernel; classes whose default
                             is one thread per quartet stay FP64), contributions summed
                             and accumulated in FP64 */
+  int natom;             /* atoms in grad (0: not given).  With 1 <= natom <= 256 the
+                            one-thread-per-quartet kernels sum a work-group's
+                            contributions in LDS and add each atom's once */
 } qc_grad2e_args;
 
 /* 1/2 sum_quartets f sum_abcd G_abcd d(ab|cd)/dR for one class pair,

@@ -95,6 +95,30 @@ __host__ __device__ constexpr int pair_terms(int la, int lb) {
   return m;
 }
 
+// sum_j c[j] R[d[j]] over a padded term list: every offset first, then
+// every R value, then the products (the loads of a list are in flight
+// together instead of one load-wait-FMA step per term)
+template <int N, typename Real>
+__device__ __forceinline__ Real term_sum(const Real* c, const int* d, const Real* R) {
+  int dv[N];
+  Real rv[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) dv[j] = d[j];
+#pragma unroll
+  for (int j = 0; j < N; ++j) rv[j] = R[dv[j]];
+  Real s = (Real)0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) s += c[j] * rv[j];
+  return s;
+}
+
+// A padded term list of one Cartesian pair: coefficients, then offsets.
+template <int N, typename Real>
+struct TermList {
+  Real c[N];
+  int d[N];
+};
+
 #ifndef QC_EW_KC_BUDGET
 #define QC_EW_KC_BUDGET 2560
 #endif
@@ -128,9 +152,9 @@ struct EriWave {
   static constexpr int IPL = (NAB * NCD + 63) / 64;  // block elements per lane
   static constexpr int WR = sizeof(Real) == 8 ? 2 : 1;  // Real per 8-byte word, inverse
   // 8-byte words: fixed, per ket primitive of a batch
-  static constexpr int FIX = (3 * EBS + NAB * TB) / (3 - WR) + (NAB * TB) / 2 + 4;
+  static constexpr int FIX = (3 * EBS + NAB * TB) / (3 - WR) + (NAB * (TB + 1)) / 2 + NHB / 2 + 5;
   static constexpr int KPER = (3 * EKS + 2 * NHR + RB * RB * RB + (L + 1) + 4 + NCD * TK +
-                               NCD * NHB) / (3 - WR) + (NCD * TK) / 2 + 2;
+                               NCD * NHB) / (3 - WR) + (NCD * (TK + 1)) / 2 + 2;
   static constexpr int BLK = NAB * NCD / (3 - WR) + 1;
   static constexpr int lds(int kc) { return FIX + (kc * KPER > BLK ? kc * KPER : BLK); }
   static constexpr int pick_kc() {
@@ -140,23 +164,25 @@ struct EriWave {
   }
   static constexpr int KC = pick_kc() >= QC_EW_KC_MIN ? pick_kc() : 1;
   struct PerKet {
+    // first in LDS, so that a list's coefficients and offsets sit at small
+    // immediate offsets from one base address; padded to TK terms with zero
+    // coefficients
+    TermList<TK, Real> kt[KC][NCD];
     Real ek[KC][3][EKS];
     Real r[2][KC][NHR];
     Real rc[KC][RB * RB * RB];
     Real f[KC][L + 1];
     Real misc[KC][4];  // X, Y, Z, pref
-    Real kc[KC][NCD][TK];  // padded to TK terms with zero coefficients
-    int kd[KC][NCD][TK];
     Real hb[KC][NCD][NHB];
   };
   struct Shared {
-    Real eb[3][EBS];
-    Real bc[NAB][TB];  // padded to TB terms with zero coefficients
-    int bi[NAB][TB];
     union {
       PerKet k;
       Real blk[NAB * NCD];  // the contracted Cartesian block, after the primitive loops
     } u;
+    Real eb[3][EBS];
+    TermList<TB, Real> bt[NAB];  // padded to TB terms with zero coefficients
+    int hbox[NHB];  // R-box offset of the (t, u, v) of Hermite index k
   };
 };
 
@@ -181,6 +207,10 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
   const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
   const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
 
+  for (int k = lane; k < NHB; k += 64) {
+    const int tuv = kHermDecode<LAB>.tuv[k];
+    sh.hbox[k] = ((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16);
+  }
   Real acc[IPL];
 #pragma unroll
   for (int j = 0; j < IPL; ++j) acc[j] = (Real)0;
@@ -210,15 +240,15 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
         for (int u = 0; u <= ay + by; ++u) {
           const Real exy = ex * eidx<LA, LB, Real>(sh.eb[1], ay, by, u);
           for (int v = 0; v <= az + bz; ++v) {
-            sh.bc[ab][n] = exy * eidx<LA, LB, Real>(sh.eb[2], az, bz, v);
-            sh.bi[ab][n] = herm_index(t, u, v);
+            sh.bt[ab].c[n] = exy * eidx<LA, LB, Real>(sh.eb[2], az, bz, v);
+            sh.bt[ab].d[n] = herm_index(t, u, v);
             ++n;
           }
         }
       }
       for (; n < TB; ++n) {
-        sh.bc[ab][n] = (Real)0;
-        sh.bi[ab][n] = 0;
+        sh.bt[ab].c[n] = (Real)0;
+        sh.bt[ab].d[n] = 0;
       }
     }
     for (int ik0 = 0; ik0 < npk; ik0 += KC) {
@@ -271,15 +301,15 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
             const Real exy = ex * eidx<LC, LD, Real>(K.ek[kk][1], cy, dy, uu);
             for (int vv = 0; vv <= cz + dz; ++vv) {
               const Real e = pref * exy * eidx<LC, LD, Real>(K.ek[kk][2], cz, dz, vv);
-              K.kc[kk][cd][n] = ((tt + uu + vv) & 1) ? -e : e;
-              K.kd[kk][cd][n] = (tt * RB + uu) * RB + vv;
+              K.kt[kk][cd].c[n] = ((tt + uu + vv) & 1) ? -e : e;
+              K.kt[kk][cd].d[n] = (tt * RB + uu) * RB + vv;
               ++n;
             }
           }
         }
         for (; n < TK; ++n) {
-          K.kc[kk][cd][n] = (Real)0;
-          K.kd[kk][cd][n] = 0;
+          K.kt[kk][cd].c[n] = (Real)0;
+          K.kt[kk][cd].d[n] = 0;
         }
       }
       // R^0_tuv (t+u+v <= L) per ket primitive of the batch, levels n = L
@@ -311,13 +341,7 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
       for (int w = lane; w < nk * NCD * NHB; w += 64) {
         const int kk = w / (NCD * NHB), r0 = w - kk * NCD * NHB;
         const int cd = r0 / NHB, k = r0 - cd * NHB;
-        const int tuv = kHermDecode<LAB>.tuv[k];
-        const Real* Rb = K.rc[kk] + ((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16);
-        // fixed trip count: every offset load issues before the first R load
-        Real h = (Real)0;
-#pragma unroll
-        for (int j = 0; j < TK; ++j) h += K.kc[kk][cd][j] * Rb[K.kd[kk][cd][j]];
-        K.hb[kk][cd][k] = h;
+        K.hb[kk][cd][k] = term_sum<TK>(K.kt[kk][cd].c, K.kt[kk][cd].d, K.rc[kk] + sh.hbox[k]);
       }
       __syncthreads();
       // V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv] for this lane's elements
@@ -327,9 +351,7 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
         if (it < NAB * NCD) {
           const int ab = it / NCD, cd = it - ab * NCD;
           Real s = (Real)0;
-          for (int kk = 0; kk < nk; ++kk)
-#pragma unroll
-            for (int m = 0; m < TB; ++m) s += sh.bc[ab][m] * K.hb[kk][cd][sh.bi[ab][m]];
+          for (int kk = 0; kk < nk; ++kk) s += term_sum<TB>(sh.bt[ab].c, sh.bt[ab].d, K.hb[kk][cd]);
           acc[j] += s;
         }
       }

@@ -245,6 +245,8 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   a.nk = in->nk;
   a.cx = in->cx;
   a.grad = in->grad;
+  if (in->natom < 0) return set_error("qc_grad2e: natom %d", in->natom);
+  a.natom = in->natom;
   a.schwarz = in->schwarz;
   a.tau_density = in->tau_density;
   a.pairs32 = in->pairs32;
@@ -254,10 +256,11 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   a.boys64 = boys_table64();
   if (!a.boys64) return set_error("qc_grad2e: qc_init not called on the current device");
   if (in->variant < 0 || in->variant > 2) return set_error("qc_grad2e: variant %d", in->variant);
-  // default (measured on (H2O)_8): one thread per quartet for ket (s s)
-  // classes up to (dp|ss), one wave with the bra side contracted first for
-  // the rest
-  const int var = in->variant != 0 ? in->variant : (ccd == 0 && cab <= 4 ? 1 : 2);
+  // default (measured on (H2O)_8): one thread per quartet for every class
+  // that has that kernel (ket (ss) up to (dp|ss), ket (ps) up to (ds|ps)),
+  // one wave with the bra side contracted first for the rest
+  const bool has_thread = (ccd == 0 && cab <= 4) || (ccd == 1 && cab <= 3);
+  const int var = in->variant != 0 ? in->variant : (has_thread ? 1 : 2);
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
 #define QC_GC(A, B, C, D)                                                                     \

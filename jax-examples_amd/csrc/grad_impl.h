@@ -41,6 +41,7 @@ struct Grad2eArgs {
   int nk;
   double cx;
   double* grad;
+  int natom;  // atoms in grad (0: not given)
   const double* boys64;
   const double* schwarz;  // per shell pair Q (density screening), or null
   double tau_density;
@@ -71,13 +72,12 @@ constexpr int GMAXP = 9;  // ket primitive pairs per batch, at most
 // ket's C-derivative coefficients (t+u+v <= la+lb); each bra pair ab then
 // takes its A-, B- and undifferentiated E against them.
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
+__device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, int64_t gid, double (&g)[12],
+                                                      int (&at)[4]) {
   constexpr int LAB = LA + LB, LCD = LC + LD, L1 = LAB + LCD + 1;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = NA * NB, NCD = NC * ND;
   constexpr int NH1 = nherm(LAB + 1), NH0 = nherm(LAB);
-  const int64_t gid = flat_block() * blockDim.x + threadIdx.x;
-  if (gid >= args.nquartets) return;
   int lo = 0, hi = args.nbra - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
     for (int ab = 0; ab < NAB; ++ab)
 #pragma unroll
       for (int cd = 0; cd < NCD; ++cd) gmax = fmax(gmax, fabs(gam[ab][cd]));
-    if (gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return;
+    if (gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return false;
   }
 
   double acc[9];
@@ -254,15 +254,54 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
   }
   const double f = (args.sp_diag[bp] ? 1.0 : 2.0) * (args.sp_diag[kp] ? 1.0 : 2.0) *
                    (bp == kp ? 1.0 : 2.0) * 0.5;
-  const int A = args.sp_atom[2 * bp], B = args.sp_atom[2 * bp + 1];
-  const int C = args.sp_atom[2 * kp], D = args.sp_atom[2 * kp + 1];
+  at[0] = args.sp_atom[2 * bp];
+  at[1] = args.sp_atom[2 * bp + 1];
+  at[2] = args.sp_atom[2 * kp];
+  at[3] = args.sp_atom[2 * kp + 1];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const double gA = f * acc[k], gP = f * acc[3 + k], gC = f * acc[6 + k];
-    atomicAdd(&args.grad[3 * A + k], gA);
-    atomicAdd(&args.grad[3 * B + k], gP - gA);
-    atomicAdd(&args.grad[3 * C + k], gC);
-    atomicAdd(&args.grad[3 * D + k], -(gP + gC));
+    g[k] = gA;
+    g[3 + k] = gP - gA;
+    g[6 + k] = gC;
+    g[9 + k] = -(gP + gC);
+  }
+  return true;
+}
+
+// atoms whose gradient a thread-per-quartet work-group sums in LDS before
+// adding it to the global gradient (one global add per touched component
+// and work-group instead of twelve per quartet)
+constexpr int GT_LDS_ATOMS = 256;
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
+  __shared__ double gsh[3 * GT_LDS_ATOMS];
+  const bool staged = args.natom > 0 && args.natom <= GT_LDS_ATOMS;  // uniform
+  const int n3 = 3 * args.natom;
+  if (staged) {
+    for (int i = threadIdx.x; i < n3; i += blockDim.x) gsh[i] = 0.0;
+    __syncthreads();
+  }
+  const int64_t gid = flat_block() * blockDim.x + threadIdx.x;
+  double g[12];
+  int at[4];
+  const bool on = gid < args.nquartets && grad2e_thread_quartet<LA, LB, LC, LD>(args, gid, g, at);
+  if (staged) {
+    if (on) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) atomicAdd(&gsh[3 * at[i] + k], g[3 * i + k]);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n3; i += blockDim.x)
+      if (gsh[i] != 0.0) atomicAdd(&args.grad[i], gsh[i]);
+  } else if (on) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) atomicAdd(&args.grad[3 * at[i] + k], g[3 * i + k]);
   }
 }
 
@@ -294,7 +333,10 @@ hipError_t grad2e_launch_thread(const Grad2eArgs& a, hipStream_t s) {
 // chunks sized so that the LDS stays under GW_BUDGET doubles.
 // ---------------------------------------------------------------------------
 
-constexpr int GW_BUDGET = 6144;     // doubles of LDS per wave (48 KB)
+#ifndef QC_GW_BUDGET
+#define QC_GW_BUDGET 6144
+#endif
+constexpr int GW_BUDGET = QC_GW_BUDGET;  // doubles of LDS per wave (48 KB)
 constexpr int GW_KC_BUDGET = 2560;  // batching ket primitives stops at 20 KB (8 waves per CU)
 
 // Largest number of Hermite terms of one ket Cartesian pair (with one extra
@@ -330,8 +372,8 @@ struct GradWave {
   // LDS bytes: fixed part, per ket pair of a chunk, per ket primitive of a
   // batch, per (ket pair, ket primitive)
   static constexpr int RS = sizeof(Real);
-  static constexpr int FIX = (NAB * NCD + 3 * EBS) * RS + 64;
-  static constexpr int YPER = (NHB0 + 3 * NHB1) * RS;
+  static constexpr int FIX = (NAB * NCD + 3 * EBS) * RS + NHB1 * 4 + 64;
+  static constexpr int YPER = 4 * NHB1 * RS;
   static constexpr int KPER = (3 * EKS + 2 * NHR + RB * RB * RB + (L1 + 1) + 4) * RS;
   static constexpr int TPER = 4 * TMAX * (RS + 4);
   static constexpr int lds(int nc, int kc) { return FIX + nc * YPER + kc * (KPER + nc * TPER); }
@@ -350,29 +392,53 @@ struct GradWave {
   static constexpr int pick_nc() {
     int nc = NCD;
     while (nc > 1 && lds(nc, KC) > 8 * GW_BUDGET) --nc;
-    return nc;
+    const int nchunk = (NCD + nc - 1) / nc;
+    return (NCD + nchunk - 1) / nchunk;  // chunks of equal size
   }
   static constexpr int NCDC = pick_nc();
+  // one ket primitive's term lists for one ket pair cd: coefficients (sign,
+  // pref and derivative factors folded in) and offsets in the R box; list 0
+  // padded with zero terms to T0, lists 1-3 to TMAX, so that the term loops
+  // have compile-time trip counts
+  struct Terms {
+    Real c[4][TMAX];
+    int d[4][TMAX];
+  };
   struct Shared {
+    // first in LDS: an item's coefficients and offsets then sit at small
+    // immediate offsets from one base address
+    Terms tl[KC][NCDC];
     Real gam[NAB * NCD];
     Real eb[3][EBS];
-    Real y0[NCDC][NHB0];
+    // per Hermite index k < NHB1 of (t, u, v): the R-box offset of (t, u, v)
+    // in bits 0-9 and the Hermite indices of (t-1, u, v), (t, u-1, v),
+    // (t, u, v-1) in bits 10-16, 17-23, 24-30 (NHB1 - 1 where the lowered
+    // index does not exist: y0 is zero there, t + u + v = la + lb + 1)
+    int hk[NHB1];
+    Real y0[NCDC][NHB1];
     Real ya[3][NCDC][NHB1];
     Real ek[KC][3][EKS];
     Real r[2][KC][NHR];
     Real rc[KC][RB * RB * RB];
     Real f[KC][L1 + 1];
     Real misc[KC][4];           // X, Y, Z, pref
-    // term coefficients (sign, pref and derivative factors folded in) and
-    // offsets in the R box; list 0 padded with zero terms to T0, lists 1-3 to
-    // TMAX, so that the term loops have compile-time trip counts
-    Real tc[KC][4][NCDC][TMAX];
-    int td[KC][4][NCDC][TMAX];
   };
 };
 
+// Waves per SIMD that the LDS use of a class allows (4 SIMDs and 160 KB of
+// LDS per CU), at most 3 (at most 2 for (dd| bras, whose Y phase spills
+// within 3 waves' 168 registers): the register budget the compiler schedules
+// for, so that registers do not lower the occupancy below what LDS allows.
 template <int LA, int LB, int LC, int LD, typename Real>
-__global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
+constexpr int grad_wave_occupancy() {
+  const int w = (int)(163840 / sizeof(typename GradWave<LA, LB, LC, LD, Real>::Shared)) / 4;
+  const int cap = LA + LB == 4 ? 2 : 3;
+  return w < 1 ? 1 : (w > cap ? cap : w);
+}
+
+template <int LA, int LB, int LC, int LD, typename Real>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(grad_wave_occupancy<LA, LB, LC, LD, Real>())))
+void grad2e_wave_kernel(Grad2eArgs args) {
   using W = GradWave<LA, LB, LC, LD, Real>;
   constexpr int LAB = W::LAB, LCD = W::LCD, L1 = W::L1, NB = W::NB, ND = W::ND;
   constexpr int NAB = W::NAB, NCD = W::NCD, NHB0 = W::NHB0, NHB1 = W::NHB1, NHR = W::NHR;
@@ -427,6 +493,16 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
     return i < 0 ? (Real)0 : sh.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 2) + t];
   };
 
+  for (int k = lane; k < NHB1; k += 64) {
+    const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
+    const int none = NHB1 - 1;
+    const int it = t > 0 ? herm_index(t - 1, u, v) : none, iu = u > 0 ? herm_index(t, u - 1, v) : none,
+              iv = v > 0 ? herm_index(t, u, v - 1) : none;
+    sh.hk[k] = ((t * RB + u) * RB + v) | (it << 10) | (iu << 17) | (iv << 24);
+  }
+  // n / d for 0 <= n < 2^11, 1 <= d <= 2^9 by one multiply: inv = ceil(2^20 / d)
+  auto div_small = [](int n, unsigned inv) -> int { return (int)(((unsigned)n * inv) >> 20); };
+
   Real acc[9];  // dA xyz, dP xyz, dC xyz
 #pragma unroll
   for (int i = 0; i < 9; ++i) acc[i] = (Real)0;
@@ -448,31 +524,46 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
     for (int cd0 = 0; cd0 < NCD; cd0 += NCDC) {
       const int ncdc = min(NCDC, NCD - cd0);
       __syncthreads();
-      // Y for this bra primitive and ket chunk
+      // Y for this bra primitive and ket chunk: per item (cd, tuv), the
+      // E columns at (t, u, v) in registers, then every ab unrolled
       for (int it = lane; it < ncdc * NHB1; it += 64) {
         const int cl = it / NHB1, k = it - cl * NHB1, cd = cd0 + cl;
-        const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
+        const int off = sh.hk[k] & 1023;
+        const int t = off / (RB * RB), u = (off / RB) % RB, v = off % RB;
+        Real ex[LA + 2][LB + 1], ey[LA + 2][LB + 1], ez[LA + 2][LB + 1];
+#pragma unroll
+        for (int i = 0; i <= LA + 1; ++i)
+#pragma unroll
+          for (int j = 0; j <= LB; ++j) {
+            ex[i][j] = sh.eb[0][(i * (LB + 1) + j) * (LAB + 2) + t];
+            ey[i][j] = sh.eb[1][(i * (LB + 1) + j) * (LAB + 2) + u];
+            ez[i][j] = sh.eb[2][(i * (LB + 1) + j) * (LAB + 2) + v];
+          }
         Real y0 = (Real)0, yx = (Real)0, yy = (Real)0, yz = (Real)0;
+#pragma unroll
         for (int ab = 0; ab < NAB; ++ab) {
           const int a = ab / NB, b = ab - a * NB;
           const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
           const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
-          if (t > ax + bx + 1 || u > ay + by + 1 || v > az + bz + 1) continue;
           const Real g = sh.gam[ab * NCD + cd];
-          const Real e0x = eb(0, ax, bx, t), e0y = eb(1, ay, by, u), e0z = eb(2, az, bz, v);
-          const Real eax = al2 * eb(0, ax + 1, bx, t) - (Real)ax * eb(0, ax - 1, bx, t);
-          const Real eay = al2 * eb(1, ay + 1, by, u) - (Real)ay * eb(1, ay - 1, by, u);
-          const Real eaz = al2 * eb(2, az + 1, bz, v) - (Real)az * eb(2, az - 1, bz, v);
-          y0 += g * e0x * e0y * e0z;
-          yx += g * eax * e0y * e0z;
-          yy += g * e0x * eay * e0z;
-          yz += g * e0x * e0y * eaz;
+          const Real e0x = ex[ax][bx], e0y = ey[ay][by], e0z = ez[az][bz];
+          // d/dA of the a factor: 2 alpha E^{a+1} - a E^{a-1}
+          Real eax = al2 * ex[ax + 1][bx], eay = al2 * ey[ay + 1][by], eaz = al2 * ez[az + 1][bz];
+          if (ax > 0) eax -= (Real)ax * ex[ax > 0 ? ax - 1 : 0][bx];
+          if (ay > 0) eay -= (Real)ay * ey[ay > 0 ? ay - 1 : 0][by];
+          if (az > 0) eaz -= (Real)az * ez[az > 0 ? az - 1 : 0][bz];
+          const Real gx = g * e0x, gyz = g * (e0y * e0z);
+          y0 += gyz * e0x;
+          yx += gyz * eax;
+          yy += gx * e0z * eay;
+          yz += gx * e0y * eaz;
         }
-        if (k < NHB0) sh.y0[cl][k] = y0;
+        sh.y0[cl][k] = y0;  // zero for t + u + v = la + lb + 1
         sh.ya[0][cl][k] = yx;
         sh.ya[1][cl][k] = yy;
         sh.ya[2][cl][k] = yz;
       }
+      const unsigned inv_nc = ((1u << 20) + ncdc - 1) / ncdc;
       for (int ik0 = 0; ik0 < npk; ik0 += KC) {
         const int nk = min(KC, npk - ik0);
         __syncthreads();
@@ -533,15 +624,15 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
               for (int vv = 0; vv <= ez2; ++vv) {
                 const Real e3 = e2 * (dim == 2 ? ga2 * ek(kk, 2, cz + 1, dz, vv) - (Real)cz * ek(kk, 2, cz - 1, dz, vv)
                                                : ek(kk, 2, cz, dz, vv));
-                sh.tc[kk][l][cl][n] = ((tt + uu + vv) & 1) ? -pref * e3 : pref * e3;
-                sh.td[kk][l][cl][n] = (tt * RB + uu) * RB + vv;
+                sh.tl[kk][cl].c[l][n] = ((tt + uu + vv) & 1) ? -pref * e3 : pref * e3;
+                sh.tl[kk][cl].d[l][n] = (tt * RB + uu) * RB + vv;
                 ++n;
               }
             }
           }
           for (const int nmax = l == 0 ? T0 : TMAX; n < nmax; ++n) {
-            sh.tc[kk][l][cl][n] = (Real)0;
-            sh.td[kk][l][cl][n] = 0;
+            sh.tl[kk][cl].c[l][n] = (Real)0;
+            sh.tl[kk][cl].d[l][n] = 0;
           }
         }
         // R^0_tuv (t+u+v <= L+1) of every primitive quartet of the batch,
@@ -570,29 +661,29 @@ __global__ __launch_bounds__(64) void grad2e_wave_kernel(Grad2eArgs args) {
           }
           __syncthreads();
         }
+        // list 0 (h) for every (ket primitive, cd, tuv), t + u + v <= la + lb + 1
         for (int it = lane; it < nk * ncdc * NHB1; it += 64) {
-          const int kk = it / (ncdc * NHB1), r0 = it - kk * ncdc * NHB1;
-          const int cl = r0 / NHB1, k = r0 - cl * NHB1;
-          const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
-          const Real* Rb = sh.rc[kk] + (t * RB + u) * RB + v;
-          Real h = (Real)0;
-#pragma unroll
-          for (int j = 0; j < T0; ++j) h += sh.tc[kk][0][cl][j] * Rb[sh.td[kk][0][cl][j]];
+          const int q = it / NHB1, k = it - q * NHB1;
+          const int kk = div_small(q, inv_nc), cl = q - kk * ncdc;
+          const int hk = sh.hk[k];
+          const Real* Rb = sh.rc[kk] + (hk & 1023);
+          const Real h = term_sum<T0>(sh.tl[kk][cl].c[0], sh.tl[kk][cl].d[0], Rb);
           acc[0] += sh.ya[0][cl][k] * h;
           acc[1] += sh.ya[1][cl][k] * h;
           acc[2] += sh.ya[2][cl][k] * h;
-          if (t > 0) acc[3] += sh.y0[cl][herm_index(t - 1, u, v)] * h;
-          if (u > 0) acc[4] += sh.y0[cl][herm_index(t, u - 1, v)] * h;
-          if (v > 0) acc[5] += sh.y0[cl][herm_index(t, u, v - 1)] * h;
-          if (k < NHB0) {
-            const Real y0 = sh.y0[cl][k];
+          acc[3] += sh.y0[cl][(hk >> 10) & 127] * h;
+          acc[4] += sh.y0[cl][(hk >> 17) & 127] * h;
+          acc[5] += sh.y0[cl][hk >> 24] * h;
+        }
+        // lists 1-3 (hC_x, hC_y, hC_z) for t + u + v <= la + lb
+        for (int it = lane; it < nk * ncdc * NHB0; it += 64) {
+          const int q = it / NHB0, k = it - q * NHB0;
+          const int kk = div_small(q, inv_nc), cl = q - kk * ncdc;
+          const Real* Rb = sh.rc[kk] + (sh.hk[k] & 1023);
+          const Real y0 = sh.y0[cl][k];
 #pragma unroll
-            for (int dim = 0; dim < 3; ++dim) {
-              Real hc = (Real)0;
-#pragma unroll
-              for (int j = 0; j < TMAX; ++j) hc += sh.tc[kk][1 + dim][cl][j] * Rb[sh.td[kk][1 + dim][cl][j]];
-              acc[6 + dim] += y0 * hc;
-            }
+          for (int dim = 0; dim < 3; ++dim) {
+            acc[6 + dim] += y0 * term_sum<TMAX>(sh.tl[kk][cl].c[1 + dim], sh.tl[kk][cl].d[1 + dim], Rb);
           }
         }
       }

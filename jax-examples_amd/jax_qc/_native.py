@@ -46,7 +46,7 @@ class QcGrad2eArgs(ctypes.Structure):
         ("kstart", c_void_p), ("nbra", c_int), ("nquartets", c_i64), ("ncart", c_i64),
         ("dj", c_void_p), ("dk0", c_void_p), ("dk1", c_void_p), ("nk", c_int),
         ("cx", c_double), ("grad", c_void_p), ("variant", c_int), ("schwarz", c_void_p),
-        ("tau_density", c_double), ("pairs32", c_void_p), ("fp32", c_int),
+        ("tau_density", c_double), ("pairs32", c_void_p), ("fp32", c_int), ("natom", c_int),
     ]
 
 

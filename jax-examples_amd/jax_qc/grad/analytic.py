@@ -129,6 +129,7 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
             a.nk = len(Dkc)
             a.cx = cx
             a.grad = grad.data_ptr()
+            a.natom = nat_
             a.variant = grad_variant_for(L.cab, L.ccd)
             a.fp32 = L.fp32
             if L.fp32:
