@@ -208,9 +208,8 @@ typedef struct {
   const double* schwarz; /* optional (with tau_density > 0): per shell pair Q */
   double tau_density;    /* skip a quartet when max|G_abcd| Q_ab Q_cd < tau_density */
   const float* pairs32;  /* fp32 = 1: FP32 primitive-pair table */
-  int fp32;              /* 1: primitive sums in FP32 (wave kernel; classes whose default
-                            is one thread per quartet stay FP64), contributions summed
-                            and accumulated in FP64 */
+  int fp32;              /* 1: primitive sums in FP32 (either kernel form), each
+                            quartet's contributions summed and accumulated in FP64 */
   int natom;             /* atoms in grad (0: not given).  With 1 <= natom <= 256 the
                             one-thread-per-quartet kernels sum a work-group's
                             contributions in LDS and add each atom's once */

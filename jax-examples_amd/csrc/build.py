@@ -81,6 +81,10 @@ def _generate():
                 f"hipError_t grad2e_launcht_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
                    if tag in GRAD_PER_THREAD else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
+                + "}\n"
+                f"hipError_t grad2e_launcht32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float>(a, s);\n"
+                   if tag in GRAD_PER_THREAD else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
                 + "}\n}  // namespace qcmd\n")
         if not os.path.exists(path) or open(path).read() != text:
             with open(path, "w") as fh:

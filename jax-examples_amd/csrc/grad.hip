@@ -180,6 +180,7 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
 
 #define QC_GDECL(A, B, C, D)                                                     \
   hipError_t grad2e_launcht_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);     \
+  hipError_t grad2e_launcht32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);   \
   hipError_t grad2e_launchw_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);   \
   hipError_t grad2e_launchw32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
 QC_ERI_CLASSES(QC_GDECL)
@@ -265,7 +266,7 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   hipError_t e = hipErrorInvalidValue;
 #define QC_GC(A, B, C, D)                                                                     \
   if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D))                    \
-    e = var == 1 ? grad2e_launcht_##A##B##C##D(a, s)                                           \
+    e = var == 1 ? (in->fp32 ? grad2e_launcht32_##A##B##C##D(a, s) : grad2e_launcht_##A##B##C##D(a, s))  \
                  : (in->fp32 ? grad2e_launchw32_##A##B##C##D(a, s) : grad2e_launchw_##A##B##C##D(a, s));
   QC_ERI_CLASSES(QC_GC)
 #undef QC_GC

@@ -71,7 +71,7 @@ constexpr int GMAXP = 9;  // ket primitive pairs per batch, at most
 // integrals h0_tuv (t+u+v <= la+lb+1) and, per direction, hC_tuv with the
 // ket's C-derivative coefficients (t+u+v <= la+lb); each bra pair ab then
 // takes its A-, B- and undifferentiated E against them.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename Real>
 __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, int64_t gid, double (&g)[12],
                                                       int (&at)[4]) {
   constexpr int LAB = LA + LB, LCD = LC + LD, L1 = LAB + LCD + 1;
@@ -88,14 +88,14 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
   }
   const int bp = args.bra_list[lo];
   const int kp = args.ket_list[args.kstart[lo] + (int)(gid - args.offsets[lo])];
-  const double* __restrict__ pp = args.pairs64;
+  const Real* __restrict__ pp = gpairs<Real>(args);
   const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
   const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
   const int64_t oa = args.sp_aoc[2 * bp], ob = args.sp_aoc[2 * bp + 1];
   const int64_t oc = args.sp_aoc[2 * kp], od = args.sp_aoc[2 * kp + 1];
   const int64_t nc = args.ncart;
 
-  double gam[NAB][NCD];
+  Real gam[NAB][NCD];
 #pragma unroll
   for (int ab = 0; ab < NAB; ++ab)
 #pragma unroll
@@ -106,70 +106,70 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
       if (args.nk > 1)
         k += args.dk1[mu * nc + la] * args.dk1[nu * nc + si] +
              args.dk1[mu * nc + si] * args.dk1[nu * nc + la];
-      gam[ab][cd] = args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k;
+      gam[ab][cd] = (Real)(args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k);
     }
   if (args.tau_density > 0.0) {
     double gmax = 0.0;
 #pragma unroll
     for (int ab = 0; ab < NAB; ++ab)
 #pragma unroll
-      for (int cd = 0; cd < NCD; ++cd) gmax = fmax(gmax, fabs(gam[ab][cd]));
+      for (int cd = 0; cd < NCD; ++cd) gmax = fmax(gmax, fabs((double)gam[ab][cd]));
     if (gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return false;
   }
 
-  double acc[9];
+  Real acc[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) acc[i] = 0.0;
+  for (int i = 0; i < 9; ++i) acc[i] = (Real)0;
 
   for (int ik = 0; ik < npk; ++ik) {
-    const double* Q = pp + (int64_t)(pk0 + ik) * PP_STRIDE;
-    const double q = Q[PP_P], ga2 = 2.0 * Q[PP_ALPHA];
+    const Real* Q = pp + (int64_t)(pk0 + ik) * PP_STRIDE;
+    const Real q = Q[PP_P], ga2 = (Real)2 * Q[PP_ALPHA];
     // ket tables up to (lc+1, ld): the C derivative only (D by invariance)
-    double Ek[3][LC + 2][LD + 1][LCD + 2];
-    double dEk[3][LC + 1][LD + 1][LCD + 2];
+    Real Ek[3][LC + 2][LD + 1][LCD + 2];
+    Real dEk[3][LC + 1][LD + 1][LCD + 2];
 #pragma unroll
     for (int dim = 0; dim < 3; ++dim) {
-      hermite_e<LC + 1, LD, double>(Ek[dim], Q[PP_PAX + dim], Q[PP_PBX + dim], 0.5 / q);
+      hermite_e<LC + 1, LD, Real>(Ek[dim], Q[PP_PAX + dim], Q[PP_PBX + dim], (Real)0.5 / q);
 #pragma unroll
       for (int i = 0; i <= LC; ++i)
 #pragma unroll
         for (int j = 0; j <= LD; ++j)
 #pragma unroll
           for (int t = 0; t <= LCD + 1; ++t) {
-            double v = ga2 * Ek[dim][i + 1][j][t];
-            if (i > 0) v -= (double)i * Ek[dim][i > 0 ? i - 1 : 0][j][t];
+            Real v = ga2 * Ek[dim][i + 1][j][t];
+            if (i > 0) v -= (Real)i * Ek[dim][i > 0 ? i - 1 : 0][j][t];
             dEk[dim][i][j][t] = v;
           }
     }
     for (int ib = 0; ib < npb; ++ib) {
-      const double* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
-      const double p = P[PP_P], al2 = 2.0 * P[PP_ALPHA];
+      const Real* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
+      const Real p = P[PP_P], al2 = (Real)2 * P[PP_ALPHA];
       // bra tables up to (la+1, lb) and the A derivative only: B follows from
       // the Hermite shift d/dP R_tuv = R_{t+1,u,v} (dB = dP - dA)
-      double Eb[3][LA + 2][LB + 1][LAB + 2];
-      double dEa[3][LA + 1][LB + 1][LAB + 2];
+      Real Eb[3][LA + 2][LB + 1][LAB + 2];
+      Real dEa[3][LA + 1][LB + 1][LAB + 2];
 #pragma unroll
       for (int dim = 0; dim < 3; ++dim) {
-        hermite_e<LA + 1, LB, double>(Eb[dim], P[PP_PAX + dim], P[PP_PBX + dim], 0.5 / p);
+        hermite_e<LA + 1, LB, Real>(Eb[dim], P[PP_PAX + dim], P[PP_PBX + dim], (Real)0.5 / p);
 #pragma unroll
         for (int i = 0; i <= LA; ++i)
 #pragma unroll
           for (int j = 0; j <= LB; ++j)
 #pragma unroll
             for (int t = 0; t <= LAB + 1; ++t) {
-              double va = al2 * Eb[dim][i + 1][j][t];
-              if (i > 0) va -= (double)i * Eb[dim][i > 0 ? i - 1 : 0][j][t];
+              Real va = al2 * Eb[dim][i + 1][j][t];
+              if (i > 0) va -= (Real)i * Eb[dim][i > 0 ? i - 1 : 0][j][t];
               dEa[dim][i][j][t] = va;
             }
       }
-      const double pqm = p * q, ps = p + q, alpha = pqm / ps;
-      const double X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
+      const Real pqm = p * q, ps = p + q, alpha = pqm / ps;
+      const Real X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
                    Z = P[PP_PX + 2] - Q[PP_PX + 2];
-      double F[L1 + 1];
-      boys_fn<L1, double>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
-      double R[L1 + 1][L1 + 1][L1 + 1];
-      hermite_r<L1, double>(R, F, -2.0 * alpha, X, Y, Z);
-      const double pref = 34.986836655249725693 / (pqm * sqrt(ps)) * P[PP_K] * Q[PP_K];
+      Real F[L1 + 1];
+      boys_fn<L1, Real>(alpha * (X * X + Y * Y + Z * Z), gboys<Real>(args), F);
+      Real R[L1 + 1][L1 + 1][L1 + 1];
+      hermite_r<L1, Real>(R, F, (Real)-2 * alpha, X, Y, Z);
+      const Real pref = (Real)34.986836655249725693 / (pqm * dev_sqrt(ps)) * P[PP_K] * Q[PP_K];
 
 #pragma unroll
       for (int c = 0; c < NC; ++c)
@@ -177,7 +177,7 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
         for (int d = 0; d < ND; ++d) {
           const int kx = CX[LC][c] + 0, ky = CY[LC][c], kz = CZ[LC][c];
           const int jx = CX[LD][d], jy = CY[LD][d], jz = CZ[LD][d];
-          double h0[NH1], hC[3][NH0];
+          Real h0[NH1], hC[3][NH0];
 #pragma unroll
           for (int t = 0; t <= LAB + 1; ++t)
 #pragma unroll
@@ -185,7 +185,7 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
 #pragma unroll
               for (int v = 0; v <= LAB + 1 - t - u; ++v) {
                 const bool low = t + u + v <= LAB;
-                double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+                Real s0 = (Real)0, sx = (Real)0, sy = (Real)0, sz = (Real)0;
 #pragma unroll
                 for (int tt = 0; tt <= kx + jx + 1; ++tt)
 #pragma unroll
@@ -194,9 +194,9 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
                     for (int vv = 0; vv <= kz + jz + 1; ++vv) {
                       const int over = (tt > kx + jx) + (uu > ky + jy) + (vv > kz + jz);
                       if (over > 1) continue;
-                      double r = R[t + tt][u + uu][v + vv];
+                      Real r = R[t + tt][u + uu][v + vv];
                       if ((tt + uu + vv) & 1) r = -r;
-                      const double ex0 = Ek[0][kx][jx][tt], ey0 = Ek[1][ky][jy][uu],
+                      const Real ex0 = Ek[0][kx][jx][tt], ey0 = Ek[1][ky][jy][uu],
                                    ez0 = Ek[2][kz][jz][vv];
                       if (over == 0) s0 += ex0 * ey0 * ez0 * r;
                       if (low) {
@@ -218,9 +218,9 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
             for (int b = 0; b < NB; ++b) {
               const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
               const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
-              double s[9];
+              Real s[9];
 #pragma unroll
-              for (int i = 0; i < 9; ++i) s[i] = 0.0;
+              for (int i = 0; i < 9; ++i) s[i] = (Real)0;
 #pragma unroll
               for (int t = 0; t <= ax + bx + 1; ++t)
 #pragma unroll
@@ -229,13 +229,13 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
                   for (int v = 0; v <= az + bz + 1; ++v) {
                     const int over = (t > ax + bx) + (u > ay + by) + (v > az + bz);
                     if (over > 1) continue;
-                    const double e0x = Eb[0][ax][bx][t], e0y = Eb[1][ay][by][u], e0z = Eb[2][az][bz][v];
-                    const double hv = h0[herm_index(t, u, v)];
+                    const Real e0x = Eb[0][ax][bx][t], e0y = Eb[1][ay][by][u], e0z = Eb[2][az][bz][v];
+                    const Real hv = h0[herm_index(t, u, v)];
                     if (u <= ay + by && v <= az + bz) s[0] += dEa[0][ax][bx][t] * e0y * e0z * hv;
                     if (t <= ax + bx && v <= az + bz) s[1] += e0x * dEa[1][ay][by][u] * e0z * hv;
                     if (t <= ax + bx && u <= ay + by) s[2] += e0x * e0y * dEa[2][az][bz][v] * hv;
                     if (over == 0) {
-                      const double e0 = e0x * e0y * e0z;
+                      const Real e0 = e0x * e0y * e0z;
                       // P = A + B: the bra distribution moved along x, y, z
                       s[3] += e0 * h0[herm_index(t + 1, u, v)];
                       s[4] += e0 * h0[herm_index(t, u + 1, v)];
@@ -245,7 +245,7 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
                       s[8] += e0 * hC[2][herm_index(t, u, v)];
                     }
                   }
-              const double gm = gam[a * NB + b][c * ND + d] * pref;
+              const Real gm = gam[a * NB + b][c * ND + d] * pref;
 #pragma unroll
               for (int i = 0; i < 9; ++i) acc[i] += gm * s[i];
             }
@@ -274,7 +274,7 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
 // and work-group instead of twelve per quartet)
 constexpr int GT_LDS_ATOMS = 256;
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename Real>
 __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
   __shared__ double gsh[3 * GT_LDS_ATOMS];
   const bool staged = args.natom > 0 && args.natom <= GT_LDS_ATOMS;  // uniform
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
   const int64_t gid = flat_block() * blockDim.x + threadIdx.x;
   double g[12];
   int at[4];
-  const bool on = gid < args.nquartets && grad2e_thread_quartet<LA, LB, LC, LD>(args, gid, g, at);
+  const bool on = gid < args.nquartets && grad2e_thread_quartet<LA, LB, LC, LD, Real>(args, gid, g, at);
   if (staged) {
     if (on) {
 #pragma unroll
@@ -305,11 +305,11 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
   }
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename Real = double>
 hipError_t grad2e_launch_thread(const Grad2eArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   const dim3 grid = grid_1d((a.nquartets + 127) / 128, 128);
-  hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD>), grid, dim3(128), 0, s, a);
+  hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real>), grid, dim3(128), 0, s, a);
   return hipGetLastError();
 }
 
