@@ -337,7 +337,13 @@ hipError_t grad2e_launch_thread(const Grad2eArgs& a, hipStream_t s) {
 #define QC_GW_BUDGET 6144
 #endif
 constexpr int GW_BUDGET = QC_GW_BUDGET;  // doubles of LDS per wave (48 KB)
-constexpr int GW_KC_BUDGET = 2560;  // batching ket primitives stops at 20 KB (8 waves per CU)
+#ifndef QC_GW_KC_BUDGET
+#define QC_GW_KC_BUDGET 2560
+#endif
+#ifndef QC_GW_KC_MIN
+#define QC_GW_KC_MIN 7
+#endif
+constexpr int GW_KC_BUDGET = QC_GW_KC_BUDGET;  // batching ket primitives stops at 20 KB (8 waves per CU)
 
 // Largest number of Hermite terms of one ket Cartesian pair (with one extra
 // order in some direction for the C-derivative lists).
@@ -374,28 +380,48 @@ struct GradWave {
   static constexpr int RS = sizeof(Real);
   static constexpr int FIX = (NAB * NCD + 3 * EBS) * RS + NHB1 * 4 + 64;
   static constexpr int YPER = 4 * NHB1 * RS;
-  static constexpr int KPER = (3 * EKS + 2 * NHR + RB * RB * RB + (L1 + 1) + 4) * RS;
+  static constexpr int KPER = (2 * NHR + RB * RB * RB) * RS;  // R levels and box
+  static constexpr int EPER = (3 * EKS + (L1 + 1) + 4) * RS;   // ket E tables, F_n, X Y Z pref
   static constexpr int TPER = 4 * TMAX * (RS + 4);
-  static constexpr int lds(int nc, int kc) { return FIX + nc * YPER + kc * (KPER + nc * TPER); }
+  static constexpr int lds(int nc, int kc, int ke) {
+    return FIX + nc * YPER + kc * (KPER + nc * TPER) + ke * EPER;
+  }
   // as many ket primitives per batch (at most GMAXP) as keep the whole ket
   // side within GW_KC_BUDGET (occupancy); ket pairs are then chunked to fit
   // GW_BUDGET.  A batch shares the barrier-separated phases (ket E, Boys,
   // term lists, R levels) among its primitive quartets.
   static constexpr int pick_kc() {
     int kc = GMAXP;
-    while (kc > 1 && lds(NCD, kc) > 8 * GW_KC_BUDGET) --kc;
+    while (kc > 1 && lds(NCD, kc, kc) > 8 * GW_KC_BUDGET) --kc;
     return kc;
   }
   // batches of 2-6 cost more in occupancy than they save in barriers
   // (measured): below 7 the kernel takes one ket primitive at a time
-  static constexpr int KC = pick_kc() >= 7 ? pick_kc() : 1;
+  static constexpr int KC = pick_kc() >= QC_GW_KC_MIN ? pick_kc() : 1;
   static constexpr int pick_nc() {
     int nc = NCD;
-    while (nc > 1 && lds(nc, KC) > 8 * GW_BUDGET) --nc;
+    while (nc > 1 && lds(nc, KC, KC) > 8 * GW_BUDGET) --nc;
     const int nchunk = (NCD + nc - 1) / nc;
     return (NCD + nchunk - 1) / nchunk;  // chunks of equal size
   }
   static constexpr int NCDC = pick_nc();
+  // The ket-side scalars (E tables, (-2 alpha)^n F_n, X Y Z pref: a few
+  // lanes' serial work per ket primitive) are built for KE ket primitives
+  // at once.  With KC = 1, KE is the largest count (at most GMAXP) that keeps
+  // the work-groups per CU (160 KB of LDS in 512-byte granules; the estimate
+  // adds 1 KB for alignment padding) at min(that of KE = 1, 8), except for
+  // (pp) kets, where batching measured slower in every class ((pp|pp),
+  // (ds|pp), (dp|pp), (dd|pp)) and KE = 1.
+  static constexpr int occ(int b) { return 163840 / ((b + 1024 + 511) / 512 * 512); }
+  static constexpr int pick_ke() {
+    if (KC > 1) return KC;
+    if (LC == 1 && LD == 1) return 1;
+    const int want = occ(lds(NCDC, 1, 1)) < 8 ? occ(lds(NCDC, 1, 1)) : 8;
+    int ke = GMAXP;
+    while (ke > 1 && occ(lds(NCDC, 1, ke)) < want) --ke;
+    return ke;
+  }
+  static constexpr int KE = pick_ke();
   // one ket primitive's term lists for one ket pair cd: coefficients (sign,
   // pref and derivative factors folded in) and offsets in the R box; list 0
   // padded with zero terms to T0, lists 1-3 to TMAX, so that the term loops
@@ -417,11 +443,11 @@ struct GradWave {
     int hk[NHB1];
     Real y0[NCDC][NHB1];
     Real ya[3][NCDC][NHB1];
-    Real ek[KC][3][EKS];
+    Real ek[KE][3][EKS];
     Real r[2][KC][NHR];
     Real rc[KC][RB * RB * RB];
-    Real f[KC][L1 + 1];
-    Real misc[KC][4];           // X, Y, Z, pref
+    Real f[KE][L1 + 1];
+    Real misc[KE][4];           // X, Y, Z, pref
   };
 };
 
@@ -431,7 +457,7 @@ struct GradWave {
 // for, so that registers do not lower the occupancy below what LDS allows.
 template <int LA, int LB, int LC, int LD, typename Real>
 constexpr int grad_wave_occupancy() {
-  const int w = (int)(163840 / sizeof(typename GradWave<LA, LB, LC, LD, Real>::Shared)) / 4;
+  const int w = (int)(163840 / ((sizeof(typename GradWave<LA, LB, LC, LD, Real>::Shared) + 511) / 512 * 512)) / 4;
   const int cap = LA + LB == 4 ? 2 : 3;
   return w < 1 ? 1 : (w > cap ? cap : w);
 }
@@ -442,7 +468,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
   using W = GradWave<LA, LB, LC, LD, Real>;
   constexpr int LAB = W::LAB, LCD = W::LCD, L1 = W::L1, NB = W::NB, ND = W::ND;
   constexpr int NAB = W::NAB, NCD = W::NCD, NHB0 = W::NHB0, NHB1 = W::NHB1, NHR = W::NHR;
-  constexpr int NCDC = W::NCDC, KC = W::KC, RB = W::RB, TMAX = W::TMAX, T0 = W::T0;
+  constexpr int NCDC = W::NCDC, KC = W::KC, KE = W::KE, RB = W::RB, TMAX = W::TMAX, T0 = W::T0;
   __shared__ typename W::Shared sh;
   const HermTable<L1>& HT = kHermTable<L1>;
   const HermDecode<LAB + 1>& HD = kHermDecode<LAB + 1>;
@@ -564,13 +590,14 @@ void grad2e_wave_kernel(Grad2eArgs args) {
         sh.ya[2][cl][k] = yz;
       }
       const unsigned inv_nc = ((1u << 20) + ncdc - 1) / ncdc;
-      for (int ik0 = 0; ik0 < npk; ik0 += KC) {
-        const int nk = min(KC, npk - ik0);
+      for (int ie0 = 0; ie0 < npk; ie0 += KE) {
+        const int ne = min(KE, npk - ie0);
         __syncthreads();
-        // ket E tables (lanes 3 kk + dim) and Boys values (lanes 32 + kk)
-        if (lane < 3 * nk) {
-          const int kk = lane / 3, dim = lane - 3 * kk;
-          const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
+        // ket E tables (lanes 3 e + dim) and Boys values (lanes 32 + e) of
+        // KE ket primitives
+        if (lane < 3 * ne) {
+          const int e = lane / 3, dim = lane - 3 * e;
+          const Real* Q = pp + (int64_t)(pk0 + ie0 + e) * PP_STRIDE;
           Real E[LC + 2][LD + 1][LCD + 2];
           hermite_e<LC + 1, LD, Real>(E, Q[PP_PAX + dim], Q[PP_PBX + dim], (Real)0.5 / Q[PP_P]);
 #pragma unroll
@@ -578,11 +605,11 @@ void grad2e_wave_kernel(Grad2eArgs args) {
 #pragma unroll
             for (int j = 0; j <= LD; ++j)
 #pragma unroll
-              for (int t = 0; t <= LCD + 1; ++t) sh.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 2) + t] = E[i][j][t];
+              for (int t = 0; t <= LCD + 1; ++t) sh.ek[e][dim][(i * (LD + 1) + j) * (LCD + 2) + t] = E[i][j][t];
         }
-        if (lane >= 32 && lane - 32 < nk) {
-          const int kk = lane - 32;
-          const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
+        if (lane >= 32 && lane - 32 < ne) {
+          const int e = lane - 32;
+          const Real* Q = pp + (int64_t)(pk0 + ie0 + e) * PP_STRIDE;
           const Real p = P[PP_P], q = Q[PP_P];
           const Real pqm = p * q, ps = p + q, alpha = pqm / ps;
           const Real X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
@@ -592,98 +619,101 @@ void grad2e_wave_kernel(Grad2eArgs args) {
           Real pw = (Real)1;
 #pragma unroll
           for (int n = 0; n <= L1; ++n) {
-            sh.f[kk][n] = pw * F[n];
+            sh.f[e][n] = pw * F[n];
             pw *= (Real)-2 * alpha;
           }
-          sh.misc[kk][0] = X;
-          sh.misc[kk][1] = Y;
-          sh.misc[kk][2] = Z;
-          sh.misc[kk][3] = (Real)34.986836655249725693 / (pqm * dev_sqrt(ps)) * P[PP_K] * Q[PP_K];
+          sh.misc[e][0] = X;
+          sh.misc[e][1] = Y;
+          sh.misc[e][2] = Z;
+          sh.misc[e][3] = (Real)34.986836655249725693 / (pqm * dev_sqrt(ps)) * P[PP_K] * Q[PP_K];
         }
-        __syncthreads();
-        // ket term lists per (ket primitive, list, ket pair): list 0 for h,
-        // 1 + x for hC_x; pref, the sign (-1)^{t'+u'+v'} and the derivative
-        // factors are folded into the coefficients
-        for (int w = lane; w < nk * 4 * ncdc; w += 64) {
-          const int kk = w / (4 * ncdc), r0 = w - kk * 4 * ncdc, l = r0 / ncdc, cl = r0 - l * ncdc;
-          const int cd = cd0 + cl;
-          const int c = cd / ND, d = cd - c * ND;
-          const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
-          const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
-          const Real pref = sh.misc[kk][3];
-          const Real ga2 = (Real)2 * pp[(int64_t)(pk0 + ik0 + kk) * PP_STRIDE + PP_ALPHA];
-          const int dim = l - 1;
-          const int ex2 = cx + dx + (dim == 0), ey2 = cy + dy + (dim == 1), ez2 = cz + dz + (dim == 2);
-          int n = 0;
-          for (int tt = 0; tt <= ex2; ++tt) {
-            const Real e1 = dim == 0 ? ga2 * ek(kk, 0, cx + 1, dx, tt) - (Real)cx * ek(kk, 0, cx - 1, dx, tt)
-                                     : ek(kk, 0, cx, dx, tt);
-            for (int uu = 0; uu <= ey2; ++uu) {
-              const Real e2 = e1 * (dim == 1 ? ga2 * ek(kk, 1, cy + 1, dy, uu) - (Real)cy * ek(kk, 1, cy - 1, dy, uu)
-                                             : ek(kk, 1, cy, dy, uu));
-              for (int vv = 0; vv <= ez2; ++vv) {
-                const Real e3 = e2 * (dim == 2 ? ga2 * ek(kk, 2, cz + 1, dz, vv) - (Real)cz * ek(kk, 2, cz - 1, dz, vv)
-                                               : ek(kk, 2, cz, dz, vv));
-                sh.tl[kk][cl].c[l][n] = ((tt + uu + vv) & 1) ? -pref * e3 : pref * e3;
-                sh.tl[kk][cl].d[l][n] = (tt * RB + uu) * RB + vv;
-                ++n;
+        for (int ik0 = ie0; ik0 < ie0 + ne; ik0 += KC) {
+          const int nk = min(KC, ie0 + ne - ik0), kb = ik0 - ie0;
+          __syncthreads();
+          // ket term lists per (ket primitive, list, ket pair): list 0 for h,
+          // 1 + x for hC_x; pref, the sign (-1)^{t'+u'+v'} and the derivative
+          // factors are folded into the coefficients
+          for (int w = lane; w < nk * 4 * ncdc; w += 64) {
+            const int kk = w / (4 * ncdc), r0 = w - kk * 4 * ncdc, l = r0 / ncdc, cl = r0 - l * ncdc;
+            const int cd = cd0 + cl;
+            const int c = cd / ND, d = cd - c * ND;
+            const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
+            const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
+            const Real pref = sh.misc[kb + kk][3];
+            const Real ga2 = (Real)2 * pp[(int64_t)(pk0 + ik0 + kk) * PP_STRIDE + PP_ALPHA];
+            const int dim = l - 1;
+            const int ex2 = cx + dx + (dim == 0), ey2 = cy + dy + (dim == 1), ez2 = cz + dz + (dim == 2);
+            int n = 0;
+            for (int tt = 0; tt <= ex2; ++tt) {
+              const Real e1 = dim == 0 ? ga2 * ek(kb + kk, 0, cx + 1, dx, tt) - (Real)cx * ek(kb + kk, 0, cx - 1, dx, tt)
+                                       : ek(kb + kk, 0, cx, dx, tt);
+              for (int uu = 0; uu <= ey2; ++uu) {
+                const Real e2 = e1 * (dim == 1 ? ga2 * ek(kb + kk, 1, cy + 1, dy, uu) - (Real)cy * ek(kb + kk, 1, cy - 1, dy, uu)
+                                               : ek(kb + kk, 1, cy, dy, uu));
+                for (int vv = 0; vv <= ez2; ++vv) {
+                  const Real e3 = e2 * (dim == 2 ? ga2 * ek(kb + kk, 2, cz + 1, dz, vv) - (Real)cz * ek(kb + kk, 2, cz - 1, dz, vv)
+                                                 : ek(kb + kk, 2, cz, dz, vv));
+                  sh.tl[kk][cl].c[l][n] = ((tt + uu + vv) & 1) ? -pref * e3 : pref * e3;
+                  sh.tl[kk][cl].d[l][n] = (tt * RB + uu) * RB + vv;
+                  ++n;
+                }
               }
             }
-          }
-          for (const int nmax = l == 0 ? T0 : TMAX; n < nmax; ++n) {
-            sh.tl[kk][cl].c[l][n] = (Real)0;
-            sh.tl[kk][cl].d[l][n] = 0;
-          }
-        }
-        // R^0_tuv (t+u+v <= L+1) of every primitive quartet of the batch,
-        // levels n = L+1 .. 0, ping-pong buffers; level 0 also goes into the
-        // RB^3 boxes
-#pragma unroll
-        for (int n = L1; n >= 0; --n) {
-          const int nh = nherm(L1 - n);
-          for (int w = lane; w < nk * nh; w += 64) {
-            const int kk = w / nh, k = w - kk * nh;
-            Real* cur = sh.r[n & 1][kk];
-            const Real* prv = sh.r[(n + 1) & 1][kk];
-            Real v;
-            if (k == 0) {
-              v = sh.f[kk][n];
-            } else {
-              const HermRec hr = HT.rec[k];
-              v = sh.misc[kk][hr.dir] * prv[hr.src1];
-              if (hr.src2 >= 0) v += (Real)hr.mult * prv[hr.src2];
-            }
-            cur[k] = v;
-            if (n == 0) {
-              const int tuv = kHermDecode<L1>.tuv[k];
-              sh.rc[kk][((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16)] = v;
+            for (const int nmax = l == 0 ? T0 : TMAX; n < nmax; ++n) {
+              sh.tl[kk][cl].c[l][n] = (Real)0;
+              sh.tl[kk][cl].d[l][n] = 0;
             }
           }
-          __syncthreads();
-        }
-        // list 0 (h) for every (ket primitive, cd, tuv), t + u + v <= la + lb + 1
-        for (int it = lane; it < nk * ncdc * NHB1; it += 64) {
-          const int q = it / NHB1, k = it - q * NHB1;
-          const int kk = div_small(q, inv_nc), cl = q - kk * ncdc;
-          const int hk = sh.hk[k];
-          const Real* Rb = sh.rc[kk] + (hk & 1023);
-          const Real h = term_sum<T0>(sh.tl[kk][cl].c[0], sh.tl[kk][cl].d[0], Rb);
-          acc[0] += sh.ya[0][cl][k] * h;
-          acc[1] += sh.ya[1][cl][k] * h;
-          acc[2] += sh.ya[2][cl][k] * h;
-          acc[3] += sh.y0[cl][(hk >> 10) & 127] * h;
-          acc[4] += sh.y0[cl][(hk >> 17) & 127] * h;
-          acc[5] += sh.y0[cl][hk >> 24] * h;
-        }
-        // lists 1-3 (hC_x, hC_y, hC_z) for t + u + v <= la + lb
-        for (int it = lane; it < nk * ncdc * NHB0; it += 64) {
-          const int q = it / NHB0, k = it - q * NHB0;
-          const int kk = div_small(q, inv_nc), cl = q - kk * ncdc;
-          const Real* Rb = sh.rc[kk] + (sh.hk[k] & 1023);
-          const Real y0 = sh.y0[cl][k];
+          // R^0_tuv (t+u+v <= L+1) of every primitive quartet of the batch,
+          // levels n = L+1 .. 0, ping-pong buffers; level 0 also goes into the
+          // RB^3 boxes
 #pragma unroll
-          for (int dim = 0; dim < 3; ++dim) {
-            acc[6 + dim] += y0 * term_sum<TMAX>(sh.tl[kk][cl].c[1 + dim], sh.tl[kk][cl].d[1 + dim], Rb);
+          for (int n = L1; n >= 0; --n) {
+            const int nh = nherm(L1 - n);
+            for (int w = lane; w < nk * nh; w += 64) {
+              const int kk = w / nh, k = w - kk * nh;
+              Real* cur = sh.r[n & 1][kk];
+              const Real* prv = sh.r[(n + 1) & 1][kk];
+              Real v;
+              if (k == 0) {
+                v = sh.f[kb + kk][n];
+              } else {
+                const HermRec hr = HT.rec[k];
+                v = sh.misc[kb + kk][hr.dir] * prv[hr.src1];
+                if (hr.src2 >= 0) v += (Real)hr.mult * prv[hr.src2];
+              }
+              cur[k] = v;
+              if (n == 0) {
+                const int tuv = kHermDecode<L1>.tuv[k];
+                sh.rc[kk][((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16)] = v;
+              }
+            }
+            __syncthreads();
+          }
+          // list 0 (h) for every (ket primitive, cd, tuv), t + u + v <= la + lb + 1
+          for (int it = lane; it < nk * ncdc * NHB1; it += 64) {
+            const int q = it / NHB1, k = it - q * NHB1;
+            const int kk = div_small(q, inv_nc), cl = q - kk * ncdc;
+            const int hk = sh.hk[k];
+            const Real* Rb = sh.rc[kk] + (hk & 1023);
+            const Real h = term_sum<T0>(sh.tl[kk][cl].c[0], sh.tl[kk][cl].d[0], Rb);
+            acc[0] += sh.ya[0][cl][k] * h;
+            acc[1] += sh.ya[1][cl][k] * h;
+            acc[2] += sh.ya[2][cl][k] * h;
+            acc[3] += sh.y0[cl][(hk >> 10) & 127] * h;
+            acc[4] += sh.y0[cl][(hk >> 17) & 127] * h;
+            acc[5] += sh.y0[cl][hk >> 24] * h;
+          }
+          // lists 1-3 (hC_x, hC_y, hC_z) for t + u + v <= la + lb
+          for (int it = lane; it < nk * ncdc * NHB0; it += 64) {
+            const int q = it / NHB0, k = it - q * NHB0;
+            const int kk = div_small(q, inv_nc), cl = q - kk * ncdc;
+            const Real* Rb = sh.rc[kk] + (sh.hk[k] & 1023);
+            const Real y0 = sh.y0[cl][k];
+#pragma unroll
+            for (int dim = 0; dim < 3; ++dim) {
+              acc[6 + dim] += y0 * term_sum<TMAX>(sh.tl[kk][cl].c[1 + dim], sh.tl[kk][cl].d[1 + dim], Rb);
+            }
           }
         }
       }
