@@ -402,6 +402,11 @@ struct GradWave {
     int nc = NCD;
     while (nc > 1 && lds(nc, KC, KC) > 8 * GW_BUDGET) --nc;
     const int nchunk = (NCD + nc - 1) / nc;
+    // a single chunk above 32 KB is split in two when the halves fit 32 KB:
+    // (dp|dp) goes from 3 to 5 work-groups per CU and from 1.25 to 0.77 s on
+    // (H2O)_16, while a third chunk or more costs more in repeated ket-side
+    // work than it gains ((dd|dp), (dd|dd) measured slower at 32 KB)
+    if (nchunk == 1 && lds(NCD, KC, KC) > 32768 && lds((NCD + 1) / 2, KC, KC) <= 32768) return (NCD + 1) / 2;
     return (NCD + nchunk - 1) / nchunk;  // chunks of equal size
   }
   static constexpr int NCDC = pick_nc();
