@@ -1,0 +1,65 @@
+"""Occupied-subspace projectors by trace-correcting purification.
+
+Each Roothaan step needs only the new density, i.e. the projector onto the
+n_occ lowest eigenvectors of the Fock matrix in the orthonormal basis (the
+reference gets it from ``eigh``, jax_qc/scf/rhf.py and uhf.py).  Trace-
+correcting purification (TC2: A. M. N. Niklasson, Phys. Rev. B 66, 155115
+(2002)) reaches the same projector with matrix products only: map the
+spectrum into [0, 1], lowest Fock eigenvalue to 1, with Gershgorin bounds,
+X0 = (hi I - F) / (hi - lo); then repeat X <- X^2 while tr X > n_occ and
+X <- 2X - X^2 otherwise.  Every eigenvalue moves monotonically to 0 or 1 and
+tr(X - X^2) = sum lambda (1 - lambda) measures the distance to idempotency.
+A spectrum without a gap at n_occ does not converge; the callers then
+diagonalise.  On a 480-function Fock pair this is about 30 batched products
+against a batched eigensolver call that costs about 11 ms on MI355X.
+"""
+
+from __future__ import annotations
+
+from typing import Optional, Sequence
+
+import torch
+
+# off: every SCF step diagonalises, as the reference does
+ENABLED = True
+
+
+def occupied_projectors(F: torch.Tensor, n_occ: Sequence[int], tol: float = 1e-12,
+                        max_iter: int = 120, check_every: int = 4) -> Optional[torch.Tensor]:
+    """Projectors onto the n_occ[b] lowest eigenvectors of the symmetric
+    matrices F[b] (shape (B, n, n)), or None when they do not converge (no
+    gap at n_occ within max_iter products).  Converged means
+    max_b tr(X_b - X_b^2) < tol and |tr X_b - n_occ[b]| < 1e-6."""
+    B, n, _ = F.shape
+    dev, dt = F.device, F.dtype
+    nocc = torch.tensor([float(k) for k in n_occ], dtype=dt, device=dev)
+    if any(k <= 0 or k >= n for k in n_occ):
+        # empty or full occupation: the projector is 0 or I
+        out = torch.zeros_like(F)
+        for b, k in enumerate(n_occ):
+            if k >= n:
+                out[b] = torch.eye(n, dtype=dt, device=dev)
+            elif k > 0:
+                return None
+        return out
+    d = torch.diagonal(F, dim1=1, dim2=2)
+    r = F.abs().sum(dim=2) - d.abs()
+    lo = (d - r).min(dim=1).values
+    hi = (d + r).max(dim=1).values
+    eye = torch.eye(n, dtype=dt, device=dev)
+    X = (hi[:, None, None] * eye - F) / (hi - lo)[:, None, None]
+    for it in range(max_iter):
+        X2 = torch.bmm(X, X)
+        trX = torch.diagonal(X, dim1=1, dim2=2).sum(-1)
+        trX2 = torch.diagonal(X2, dim1=1, dim2=2).sum(-1)
+        Xn = torch.where((trX > nocc)[:, None, None], X2, 2.0 * X - X2)
+        if (it + 1) % check_every == 0:
+            idem = float((trX - trX2).abs().max())
+            if idem < tol:
+                X = 0.5 * (Xn + Xn.transpose(1, 2))
+                trn = torch.diagonal(X, dim1=1, dim2=2).sum(-1)
+                if float((trn - nocc).abs().max()) < 1e-6:
+                    return X
+                return None
+        X = Xn
+    return None

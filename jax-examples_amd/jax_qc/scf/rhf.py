@@ -3,11 +3,15 @@
 Same step order and convergence rule as the reference: Fock build ->
 optional damping -> DIIS -> diagonalise in the X basis -> density ->
 energy; converged when |dE| < scf_conv and max|e| < max(scf_conv, 1e-6).
-All matrices are FP64 device tensors; the Fock build is the hot path.
+All matrices are FP64 device tensors; the Fock build is the hot path.  The
+density of a step is the occupied projector of X^T F X, reached by
+purification (jax_qc.scf.purify; eigh when it does not converge); the
+converged state's orbitals come from one eigh of its Fock matrix.
 """
 
 from __future__ import annotations
 
+import dataclasses
 from typing import Optional
 
 import torch
@@ -21,11 +25,24 @@ from jax_qc.scf.energy import electronic_energy_rhf
 from jax_qc.scf.fock import build_fock_rhf
 from jax_qc.scf.guess import core_guess
 from jax_qc.scf.orthogonalize import symmetric_orthogonalization
+from jax_qc.scf import purify
 
 
 def _diagonalize_fock(F, X):
     e, Cp = torch.linalg.eigh(X.T @ F @ X)
     return e, X @ Cp
+
+
+def _density_step(F, X, n_occ):
+    """(D, orbital energies, C) of one Roothaan step; the last two are None
+    when the density came from purification."""
+    if purify.ENABLED:
+        Xt = X.T.contiguous()
+        P = purify.occupied_projectors((Xt @ F @ X)[None], [n_occ])
+        if P is not None:
+            return 2.0 * (X @ P[0] @ Xt), None, None
+    orb_e, C = _diagonalize_fock(F, X)
+    return density_rhf(C, n_occ), orb_e, C
 
 
 def scf_step_rhf(state, H_core, ERI, S, X, n_occ, diis_history, damping, timer=None):
@@ -36,9 +53,7 @@ def scf_step_rhf(state, H_core, ERI, S, X, n_occ, diis_history, damping, timer=N
     with optional_stage(timer, "diis"):
         F_ext, diis_history, error = diis_extrapolate(diis_history, F_d, state.density, S, X)
     with optional_stage(timer, "diag"):
-        orb_e, C = _diagonalize_fock(F_ext, X)
-    with optional_stage(timer, "density"):
-        D_new = density_rhf(C, n_occ)
+        D_new, orb_e, C = _density_step(F_ext, X, n_occ)
     with optional_stage(timer, "energy"):
         E = float(electronic_energy_rhf(D_new, H_core, F_ext))
     new = SCFState(density=D_new, fock=F_ext, coefficients=C, orbital_energies=orb_e, energy=E,
@@ -75,6 +90,10 @@ def run_rhf(mol: Molecule, ints: IntegralSet, config: CalcConfig = CalcConfig(),
             if dE < config.scf_conv and err_max < max(config.scf_conv, 1e-6):
                 converged = True
                 break
+        if state.coefficients is None:
+            with optional_stage(timer, "diag"):
+                e_f, C_f = _diagonalize_fock(state.fock, X)
+            state = dataclasses.replace(state, coefficients=C_f, orbital_energies=e_f)
     return SCFResult(converged=converged, state=state, energy=state.energy + float(ints.E_nuc),
                      E_elec=state.energy, E_nuc=float(ints.E_nuc), n_iterations=state.iteration,
                      S=S, H_core=H, ERI=ints.ERI)

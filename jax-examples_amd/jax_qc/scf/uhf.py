@@ -9,6 +9,7 @@ channels in ``SCFResult.uhf`` for the UHF gradient.
 
 from __future__ import annotations
 
+import dataclasses
 from dataclasses import dataclass
 
 import torch
@@ -22,6 +23,7 @@ from jax_qc.scf.energy import electronic_energy_uhf
 from jax_qc.scf.fock import build_fock_uhf
 from jax_qc.scf.guess import core_guess_uhf
 from jax_qc.scf.orthogonalize import symmetric_orthogonalization
+from jax_qc.scf import purify
 
 
 def _diagonalize_pair(Fa, Fb, X):
@@ -31,6 +33,20 @@ def _diagonalize_pair(Fa, Fb, X):
     Xt = X.T.contiguous()
     e, Cp = torch.linalg.eigh(Xt @ torch.stack([Fa, Fb]) @ X)
     return (e[0], X @ Cp[0]), (e[1], X @ Cp[1])
+
+
+def _density_pair_step(Fa, Fb, X, n_alpha, n_beta):
+    """(Da, Db, orbitals) of one UHF step; orbitals is ((e_a, C_a), (e_b,
+    C_b)), or None when the densities came from purification."""
+    if purify.ENABLED:
+        Xt = X.T.contiguous()
+        P = purify.occupied_projectors(Xt @ torch.stack([Fa, Fb]) @ X, [n_alpha, n_beta])
+        if P is not None:
+            D = X @ P @ Xt
+            return D[0], D[1], None
+    (oa, Ca), (ob, Cb) = _diagonalize_pair(Fa, Fb, X)
+    Da, Db = density_uhf(Ca, Cb, n_alpha, n_beta)
+    return Da, Db, ((oa, Ca), (ob, Cb))
 
 
 @dataclass
@@ -58,9 +74,8 @@ def scf_step_uhf(state, H, ERI, S, X, n_alpha, n_beta, hist, damping, timer=None
         Fa, Fb, hist, ea, eb = diis_extrapolate_uhf(hist, Fa, Fb, state.density_alpha,
                                                     state.density_beta, S, X)
     with optional_stage(timer, "diag"):
-        (oa, Ca), (ob, Cb) = _diagonalize_pair(Fa, Fb, X)
-    with optional_stage(timer, "density"):
-        Da, Db = density_uhf(Ca, Cb, n_alpha, n_beta)
+        Da, Db, orb = _density_pair_step(Fa, Fb, X, n_alpha, n_beta)
+    (oa, Ca), (ob, Cb) = orb if orb is not None else ((None, None), (None, None))
     with optional_stage(timer, "energy"):
         E = float(electronic_energy_uhf(Da, Db, H, Fa, Fb))
     err = max(float(ea.abs().max()), float(eb.abs().max()))
@@ -93,6 +108,11 @@ def run_uhf(mol: Molecule, ints: IntegralSet, config: CalcConfig = CalcConfig(),
             if dE < config.scf_conv and err < max(config.scf_conv, 1e-6):
                 converged = True
                 break
+        if state.coefficients_alpha is None:
+            with optional_stage(timer, "diag"):
+                (oa, Ca), (ob, Cb) = _diagonalize_pair(state.fock_alpha, state.fock_beta, X)
+            state = dataclasses.replace(state, coefficients_alpha=Ca, coefficients_beta=Cb,
+                                        orbital_energies_alpha=oa, orbital_energies_beta=ob)
     total = state.density_alpha + state.density_beta
     st = SCFState(density=total, fock=state.fock_alpha, coefficients=state.coefficients_alpha,
                   orbital_energies=state.orbital_energies_alpha, energy=state.energy,
