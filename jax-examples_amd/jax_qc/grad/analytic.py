@@ -29,6 +29,10 @@ from jax_qc.integrals import engine as eng
 # GRAD_VARIANT_OVERRIDE to force one form for every class that has it.
 GRAD_VARIANTS: dict = {}
 GRAD_VARIANT_OVERRIDE = None
+# pass the atom count to qc_grad2e so that the thread-per-quartet kernels sum
+# a work-group's contributions in LDS (False: every add goes to the global
+# gradient, the form used above 256 atoms)
+GRAD_STAGE_IN_LDS = True
 
 
 def grad_variant_for(cab: int, ccd: int) -> int:
@@ -129,7 +133,7 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
             a.nk = len(Dkc)
             a.cx = cx
             a.grad = grad.data_ptr()
-            a.natom = nat_
+            a.natom = nat_ if GRAD_STAGE_IN_LDS else 0
             a.variant = grad_variant_for(L.cab, L.ccd)
             a.fp32 = L.fp32
             if L.fp32:

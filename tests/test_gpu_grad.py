@@ -134,3 +134,26 @@ def test_kernel_forms_fp32_split_and_density_screening():
         an.GRAD_VARIANT_OVERRIDE = None
     gd = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt, tau_density=1e-11).cpu().numpy()
     np.testing.assert_allclose(gd, g[2], atol=1e-7)
+
+
+def test_thread_kernels_lds_staged_adds_equal_global_adds():
+    """The thread-per-quartet kernels' per-work-group LDS sums (natom <= 256)
+    give the gradient that direct global atomic adds give (natom = 0)."""
+    import jax_qc
+    from jax_qc.grad import analytic as an
+
+    mol = M.water_cluster((2, 1, 1), seed=1)
+    bs = build_basis_set(mol, "even-tempered")
+    ints = jax_qc.compute_integrals(mol, bs, eri="store", tau=1e-14)
+    res = jax_qc.run_scf(mol, ints, CalcConfig(method="rhf", scf_conv=1e-12))
+    pt = ints.ERI.pt
+    g = {}
+    try:
+        an.GRAD_VARIANT_OVERRIDE = 1
+        for staged in (True, False):
+            an.GRAD_STAGE_IN_LDS = staged
+            g[staged] = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt).cpu().numpy()
+    finally:
+        an.GRAD_VARIANT_OVERRIDE = None
+        an.GRAD_STAGE_IN_LDS = True
+    np.testing.assert_allclose(g[True], g[False], atol=1e-12)
