@@ -329,7 +329,8 @@ hipError_t grad2e_launch_thread(const Grad2eArgs& a, hipStream_t s) {
 //   dC_x  = pref sum_cd sum_tuv Y0_cd[tuv] hC_cd,x[tuv]
 // then dB = dP - dA and dD = -(dP + dC) (translational invariance).  Y is
 // built once per bra primitive and reused by every ket primitive; the work
-// items (tuv, cd) are spread over all 64 lanes and ket pairs are taken in
+// items (tuv, cd) are spread over all lanes of the work-group (one or two
+// waves, NT) and ket pairs are taken in
 // chunks sized so that the LDS stays under GW_BUDGET doubles.
 // ---------------------------------------------------------------------------
 
@@ -375,6 +376,11 @@ struct GradWave {
   // 1 + x for hC_x (one more term along x)
   static constexpr int TMAX = tmax_terms(LC, LD);
   static constexpr int T0 = pair_terms(LC, LD);  // list 0 has no extra order
+  // threads per quartet work-group: two waves share one quartet's LDS for
+  // (dp) and (dd) kets, whose LDS allows few work-groups per CU (measured on
+  // (H2O)_16: (dd|dp) 361 -> 294 ms, (dd|dd) 86 -> 65 ms, (dp|dp) FP32 153 ->
+  // 128 ms); one wave for lighter kets, where two lost ((ds|pp) 1.19 -> 1.81 s)
+  static constexpr int NT = LC + LD >= 3 ? 128 : 64;
   // LDS bytes: fixed part, per ket pair of a chunk, per ket primitive of a
   // batch, per (ket pair, ket primitive)
   static constexpr int RS = sizeof(Real);
@@ -453,6 +459,7 @@ struct GradWave {
     Real rc[KC][RB * RB * RB];
     Real f[KE][L1 + 1];
     Real misc[KE][4];           // X, Y, Z, pref
+    double wmax[NT / 64];  // per-wave max |G| (density screening)
   };
 };
 
@@ -462,23 +469,25 @@ struct GradWave {
 // for, so that registers do not lower the occupancy below what LDS allows.
 template <int LA, int LB, int LC, int LD, typename Real>
 constexpr int grad_wave_occupancy() {
-  const int w = (int)(163840 / ((sizeof(typename GradWave<LA, LB, LC, LD, Real>::Shared) + 511) / 512 * 512)) / 4;
+  using W = GradWave<LA, LB, LC, LD, Real>;
+  const int w = (int)(163840 / ((sizeof(typename W::Shared) + 511) / 512 * 512)) * (W::NT / 64) / 4;
   const int cap = LA + LB == 4 ? 2 : 3;
   return w < 1 ? 1 : (w > cap ? cap : w);
 }
 
 template <int LA, int LB, int LC, int LD, typename Real>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(grad_wave_occupancy<LA, LB, LC, LD, Real>())))
+__global__ __launch_bounds__((GradWave<LA, LB, LC, LD, Real>::NT)) __attribute__((amdgpu_waves_per_eu(grad_wave_occupancy<LA, LB, LC, LD, Real>())))
 void grad2e_wave_kernel(Grad2eArgs args) {
   using W = GradWave<LA, LB, LC, LD, Real>;
   constexpr int LAB = W::LAB, LCD = W::LCD, L1 = W::L1, NB = W::NB, ND = W::ND;
   constexpr int NAB = W::NAB, NCD = W::NCD, NHB0 = W::NHB0, NHB1 = W::NHB1, NHR = W::NHR;
   constexpr int NCDC = W::NCDC, KC = W::KC, KE = W::KE, RB = W::RB, TMAX = W::TMAX, T0 = W::T0;
+  constexpr int NT = W::NT;  // threads of the work-group
   __shared__ typename W::Shared sh;
   const HermTable<L1>& HT = kHermTable<L1>;
   const HermDecode<LAB + 1>& HD = kHermDecode<LAB + 1>;
 
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
   const int64_t gid = flat_block();
   if (gid >= args.nquartets) return;  // whole work-group (one wave)
   int lo = 0, hi = args.nbra - 1;
@@ -498,7 +507,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
   const int64_t oc = args.sp_aoc[2 * kp], od = args.sp_aoc[2 * kp + 1];
   const int64_t nc = args.ncart;
 
-  for (int e = lane; e < NAB * NCD; e += 64) {
+  for (int e = tid; e < NAB * NCD; e += NT) {
     const int ab = e / NCD, cd = e % NCD;
     const int64_t mu = oa + ab / NB, nu = ob + ab % NB, la = oc + cd / ND, si = od + cd % ND;
     double k = args.dk0[mu * nc + la] * args.dk0[nu * nc + si] +
@@ -509,11 +518,17 @@ void grad2e_wave_kernel(Grad2eArgs args) {
     sh.gam[e] = (Real)(args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k);
   }
   if (args.tau_density > 0.0) {
-    // each lane's maximum over the G entries it wrote, then over the wave
+    // each thread's maximum over the G entries it wrote, then over the wave
+    // and the work-group
     double gmax = 0.0;
-    for (int e = lane; e < NAB * NCD; e += 64) gmax = fmax(gmax, fabs((double)sh.gam[e]));
+    for (int e = tid; e < NAB * NCD; e += NT) gmax = fmax(gmax, fabs((double)sh.gam[e]));
     for (int w = 32; w > 0; w >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, w));
-    if (gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return;  // wave-uniform
+    if (NT > 64) {
+      if (lane == 0) sh.wmax[tid >> 6] = gmax;
+      __syncthreads();
+      gmax = fmax(sh.wmax[0], sh.wmax[NT / 64 - 1]);
+    }
+    if (gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return;  // uniform
   }
 
   // E^{ij}_t of the bra / ket tables in LDS (zero for t > i + j).
@@ -524,7 +539,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
     return i < 0 ? (Real)0 : sh.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 2) + t];
   };
 
-  for (int k = lane; k < NHB1; k += 64) {
+  for (int k = tid; k < NHB1; k += NT) {
     const int tuv = HD.tuv[k], t = tuv & 255, u = (tuv >> 8) & 255, v = tuv >> 16;
     const int none = NHB1 - 1;
     const int it = t > 0 ? herm_index(t - 1, u, v) : none, iu = u > 0 ? herm_index(t, u - 1, v) : none,
@@ -541,15 +556,15 @@ void grad2e_wave_kernel(Grad2eArgs args) {
   for (int ib = 0; ib < npb; ++ib) {
     const Real* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
     __syncthreads();
-    if (lane < 3) {
+    if (tid < 3) {
       Real E[LA + 2][LB + 1][LAB + 2];
-      hermite_e<LA + 1, LB, Real>(E, P[PP_PAX + lane], P[PP_PBX + lane], (Real)0.5 / P[PP_P]);
+      hermite_e<LA + 1, LB, Real>(E, P[PP_PAX + tid], P[PP_PBX + tid], (Real)0.5 / P[PP_P]);
 #pragma unroll
       for (int i = 0; i <= LA + 1; ++i)
 #pragma unroll
         for (int j = 0; j <= LB; ++j)
 #pragma unroll
-          for (int t = 0; t <= LAB + 1; ++t) sh.eb[lane][(i * (LB + 1) + j) * (LAB + 2) + t] = E[i][j][t];
+          for (int t = 0; t <= LAB + 1; ++t) sh.eb[tid][(i * (LB + 1) + j) * (LAB + 2) + t] = E[i][j][t];
     }
     const Real al2 = (Real)2 * P[PP_ALPHA];
     for (int cd0 = 0; cd0 < NCD; cd0 += NCDC) {
@@ -557,7 +572,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
       __syncthreads();
       // Y for this bra primitive and ket chunk: per item (cd, tuv), the
       // E columns at (t, u, v) in registers, then every ab unrolled
-      for (int it = lane; it < ncdc * NHB1; it += 64) {
+      for (int it = tid; it < ncdc * NHB1; it += NT) {
         const int cl = it / NHB1, k = it - cl * NHB1, cd = cd0 + cl;
         const int off = sh.hk[k] & 1023;
         const int t = off / (RB * RB), u = (off / RB) % RB, v = off % RB;
@@ -600,8 +615,8 @@ void grad2e_wave_kernel(Grad2eArgs args) {
         __syncthreads();
         // ket E tables (lanes 3 e + dim) and Boys values (lanes 32 + e) of
         // KE ket primitives
-        if (lane < 3 * ne) {
-          const int e = lane / 3, dim = lane - 3 * e;
+        if (tid < 3 * ne) {
+          const int e = tid / 3, dim = tid - 3 * e;
           const Real* Q = pp + (int64_t)(pk0 + ie0 + e) * PP_STRIDE;
           Real E[LC + 2][LD + 1][LCD + 2];
           hermite_e<LC + 1, LD, Real>(E, Q[PP_PAX + dim], Q[PP_PBX + dim], (Real)0.5 / Q[PP_P]);
@@ -612,8 +627,8 @@ void grad2e_wave_kernel(Grad2eArgs args) {
 #pragma unroll
               for (int t = 0; t <= LCD + 1; ++t) sh.ek[e][dim][(i * (LD + 1) + j) * (LCD + 2) + t] = E[i][j][t];
         }
-        if (lane >= 32 && lane - 32 < ne) {
-          const int e = lane - 32;
+        if (tid >= 32 && tid - 32 < ne) {
+          const int e = tid - 32;
           const Real* Q = pp + (int64_t)(pk0 + ie0 + e) * PP_STRIDE;
           const Real p = P[PP_P], q = Q[PP_P];
           const Real pqm = p * q, ps = p + q, alpha = pqm / ps;
@@ -638,7 +653,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
           // ket term lists per (ket primitive, list, ket pair): list 0 for h,
           // 1 + x for hC_x; pref, the sign (-1)^{t'+u'+v'} and the derivative
           // factors are folded into the coefficients
-          for (int w = lane; w < nk * 4 * ncdc; w += 64) {
+          for (int w = tid; w < nk * 4 * ncdc; w += NT) {
             const int kk = w / (4 * ncdc), r0 = w - kk * 4 * ncdc, l = r0 / ncdc, cl = r0 - l * ncdc;
             const int cd = cd0 + cl;
             const int c = cd / ND, d = cd - c * ND;
@@ -675,7 +690,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
 #pragma unroll
           for (int n = L1; n >= 0; --n) {
             const int nh = nherm(L1 - n);
-            for (int w = lane; w < nk * nh; w += 64) {
+            for (int w = tid; w < nk * nh; w += NT) {
               const int kk = w / nh, k = w - kk * nh;
               Real* cur = sh.r[n & 1][kk];
               const Real* prv = sh.r[(n + 1) & 1][kk];
@@ -696,7 +711,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
             __syncthreads();
           }
           // list 0 (h) for every (ket primitive, cd, tuv), t + u + v <= la + lb + 1
-          for (int it = lane; it < nk * ncdc * NHB1; it += 64) {
+          for (int it = tid; it < nk * ncdc * NHB1; it += NT) {
             const int q = it / NHB1, k = it - q * NHB1;
             const int kk = div_small(q, inv_nc), cl = q - kk * ncdc;
             const int hk = sh.hk[k];
@@ -710,7 +725,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
             acc[5] += sh.y0[cl][hk >> 24] * h;
           }
           // lists 1-3 (hC_x, hC_y, hC_z) for t + u + v <= la + lb
-          for (int it = lane; it < nk * ncdc * NHB0; it += 64) {
+          for (int it = tid; it < nk * ncdc * NHB0; it += NT) {
             const int q = it / NHB0, k = it - q * NHB0;
             const int kk = div_small(q, inv_nc), cl = q - kk * ncdc;
             const Real* Rb = sh.rc[kk] + (sh.hk[k] & 1023);
@@ -724,7 +739,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
       }
     }
   }
-  double accd[9];  // the wave sum in FP64 whatever the arithmetic precision
+  double accd[9];  // each wave's sum in FP64 whatever the arithmetic precision
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     accd[i] = (double)acc[i];
@@ -749,8 +764,8 @@ void grad2e_wave_kernel(Grad2eArgs args) {
 template <int LA, int LB, int LC, int LD, typename Real = double>
 hipError_t grad2e_launch_wave(const Grad2eArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
-  hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD, Real>), grid_1d(a.nquartets, 64), dim3(64),
-                     0, s, a);
+  constexpr int NT = GradWave<LA, LB, LC, LD, Real>::NT;
+  hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD, Real>), grid_1d(a.nquartets, NT), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 
