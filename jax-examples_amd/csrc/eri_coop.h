@@ -2,9 +2,9 @@
 // where one thread per quartet would need several hundred live values (R_tuv
 // alone has 165 entries at L = 8) and spills to scratch.
 //
-// One 64-lane wave per contracted quartet; every lane owns a fixed set of
-// elements of the Cartesian block V[ab][cd] and accumulates them in
-// registers.  Per bra primitive pair: per bra Cartesian pair ab the list of
+// One work-group of one or two 64-lane waves per contracted quartet; every
+// thread owns a fixed set of elements of the Cartesian block V[ab][cd] and
+// accumulates them in registers.  Per bra primitive pair: per bra Cartesian pair ab the list of
 // its nonzero E^{ab}_{tuv} (coefficient, Hermite index).  Per batch of ket
 // primitive pairs (as many as the LDS budget allows): Boys values, the
 // R^n_tuv recursion (jax_qc/integrals/mcmurchie_davidson.py:113-172) level
@@ -128,7 +128,7 @@ struct TermList {
 constexpr int EW_BUDGET = 6144;                // 8-byte words of LDS per wave
 constexpr int EW_KC_BUDGET = QC_EW_KC_BUDGET;  // batching ket primitives stops here (occupancy)
 
-// One 64-lane wave per contracted quartet.  Per bra primitive pair: the bra
+// One work-group (NT threads) per contracted quartet.  Per bra primitive pair: the bra
 // E tables and, per bra Cartesian pair ab, the list of its nonzero
 // E^{ab}_{tuv} (coefficient, Hermite index).  Per batch of ket primitive
 // pairs: ket E tables, (-2 alpha)^n F_n, the R^0_tuv box (R recursion level
@@ -149,7 +149,12 @@ struct EriWave {
   static constexpr int EKS = (LC + 1) * (LD + 1) * (LCD + 1);
   static constexpr int RB = L + 1;
   static constexpr int TB = pair_terms(LA, LB), TK = pair_terms(LC, LD);
-  static constexpr int IPL = (NAB * NCD + 63) / 64;  // block elements per lane
+  // threads per quartet work-group: two waves share one quartet's LDS for
+  // (dd|dd) (measured on (H2O)_8: FP64 2.12 -> 1.47 ms, and half the block
+  // elements per thread, 422 -> 198 VGPRs); one wave elsewhere, where two
+  // measured slower ((dp|dp) 14.6 -> 17.6 ms, (dd|dp) 7.5 -> 9.7 ms)
+  static constexpr int NT = LC + LD >= 4 ? 128 : 64;
+  static constexpr int IPL = (NAB * NCD + NT - 1) / NT;  // block elements per thread
   static constexpr int WR = sizeof(Real) == 8 ? 2 : 1;  // Real per 8-byte word, inverse
   // 8-byte words: fixed, per ket primitive of a batch
   static constexpr int FIX = (3 * EBS + NAB * TB) / (3 - WR) + (NAB * (TB + 1)) / 2 + NHB / 2 + 5;
@@ -187,7 +192,7 @@ struct EriWave {
 };
 
 template <int LA, int LB, int LC, int LD, typename Real>
-__global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
+__global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT)) void eri_coop_kernel(EriArgs args) {
   using W = EriWave<LA, LB, LC, LD, Real>;
   constexpr int LAB = W::LAB, LCD = W::LCD, L = W::L;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -197,7 +202,8 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
   __shared__ typename W::Shared sh;
   auto& K = sh.u.k;
 
-  const int lane = threadIdx.x;
+  constexpr int NT = W::NT;
+  const int tid = threadIdx.x;
   int bp, kp;
   const int64_t gid = flat_block();
   if (gid >= args.nquartets) return;  // whole work-group (one wave)
@@ -207,7 +213,7 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
   const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
   const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
 
-  for (int k = lane; k < NHB; k += 64) {
+  for (int k = tid; k < NHB; k += NT) {
     const int tuv = kHermDecode<LAB>.tuv[k];
     sh.hbox[k] = ((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16);
   }
@@ -218,19 +224,19 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
   for (int ib = 0; ib < npb; ++ib) {
     const Real* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
     __syncthreads();
-    if (lane < 3) {
+    if (tid < 3) {
       Real E[LA + 1][LB + 1][LAB + 1];
-      hermite_e<LA, LB, Real>(E, P[PP_PAX + lane], P[PP_PBX + lane], (Real)0.5 / P[PP_P]);
+      hermite_e<LA, LB, Real>(E, P[PP_PAX + tid], P[PP_PBX + tid], (Real)0.5 / P[PP_P]);
 #pragma unroll
       for (int i = 0; i <= LA; ++i)
 #pragma unroll
         for (int j = 0; j <= LB; ++j)
 #pragma unroll
-          for (int t = 0; t <= LAB; ++t) sh.eb[lane][(i * (LB + 1) + j) * (LAB + 1) + t] = E[i][j][t];
+          for (int t = 0; t <= LAB; ++t) sh.eb[tid][(i * (LB + 1) + j) * (LAB + 1) + t] = E[i][j][t];
     }
     __syncthreads();
     // bra term lists
-    for (int ab = lane; ab < NAB; ab += 64) {
+    for (int ab = tid; ab < NAB; ab += NT) {
       const int a = ab / NB, b = ab - a * NB;
       const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
       const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
@@ -254,8 +260,8 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
     for (int ik0 = 0; ik0 < npk; ik0 += KC) {
       const int nk = min(KC, npk - ik0);
       __syncthreads();
-      if (lane < 3 * nk) {
-        const int kk = lane / 3, dim = lane - 3 * kk;
+      if (tid < 3 * nk) {
+        const int kk = tid / 3, dim = tid - 3 * kk;
         const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
         Real E[LC + 1][LD + 1][LCD + 1];
         hermite_e<LC, LD, Real>(E, Q[PP_PAX + dim], Q[PP_PBX + dim], (Real)0.5 / Q[PP_P]);
@@ -266,8 +272,8 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
 #pragma unroll
             for (int t = 0; t <= LCD; ++t) K.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 1) + t] = E[i][j][t];
       }
-      if (lane >= 32 && lane - 32 < nk) {
-        const int kk = lane - 32;
+      if (tid >= 32 && tid - 32 < nk) {
+        const int kk = tid - 32;
         const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
         const Real p = P[PP_P], q = Q[PP_P];
         const Real pqm = p * q, ps = p + q, alpha = pqm / ps;
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
       }
       __syncthreads();
       // ket term lists (pref and sign folded in)
-      for (int w = lane; w < nk * NCD; w += 64) {
+      for (int w = tid; w < nk * NCD; w += NT) {
         const int kk = w / NCD, cd = w - kk * NCD;
         const int c = cd / ND, d = cd - c * ND;
         const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
@@ -317,7 +323,7 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
 #pragma unroll
       for (int n = L; n >= 0; --n) {
         const int nh = nherm(L - n);
-        for (int w = lane; w < nk * nh; w += 64) {
+        for (int w = tid; w < nk * nh; w += NT) {
           const int kk = w / nh, k = w - kk * nh;
           Real* cur = K.r[n & 1][kk];
           const Real* prv = K.r[(n + 1) & 1][kk];
@@ -338,16 +344,16 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
         __syncthreads();
       }
       // h_cd[tuv] for t+u+v <= la+lb
-      for (int w = lane; w < nk * NCD * NHB; w += 64) {
+      for (int w = tid; w < nk * NCD * NHB; w += NT) {
         const int kk = w / (NCD * NHB), r0 = w - kk * NCD * NHB;
         const int cd = r0 / NHB, k = r0 - cd * NHB;
         K.hb[kk][cd][k] = term_sum<TK>(K.kt[kk][cd].c, K.kt[kk][cd].d, K.rc[kk] + sh.hbox[k]);
       }
       __syncthreads();
-      // V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv] for this lane's elements
+      // V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv] for this thread's elements
 #pragma unroll
       for (int j = 0; j < IPL; ++j) {
-        const int it = lane + 64 * j;
+        const int it = tid + NT * j;
         if (it < NAB * NCD) {
           const int ab = it / NCD, cd = it - ab * NCD;
           Real s = (Real)0;
@@ -361,13 +367,13 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
   Real* blk = sh.u.blk;
 #pragma unroll
   for (int j = 0; j < IPL; ++j) {
-    const int it = lane + 64 * j;
+    const int it = tid + NT * j;
     if (it < NAB * NCD) blk[it] = acc[j];
   }
   __syncthreads();
 
   if (args.mode == ERI_SCHWARZ) {
-    if (lane == 0) {
+    if (tid == 0) {
       double mx = 0.0;
       for (int i = 0; i < NAB && i < NCD; ++i) {
         const double v = (double)blk[i * NCD + i];
@@ -382,7 +388,7 @@ __global__ __launch_bounds__(64) void eri_coop_kernel(EriArgs args) {
   const int ao_a = args.sp_ao[2 * bp], ao_b = args.sp_ao[2 * bp + 1];
   const int ao_c = args.sp_ao[2 * kp], ao_d = args.sp_ao[2 * kp + 1];
   const int64_t n = args.nbf;
-  for (int o = lane; o < SA * SB * SC * SD; o += 64) {
+  for (int o = tid; o < SA * SB * SC * SD; o += NT) {
     const int sa = o / (SB * SC * SD), sb = (o / (SC * SD)) % SB, sc = (o / SD) % SC, sd = o % SD;
     Real v = (Real)0;
     for (int a = 0; a < NA; ++a) {
@@ -423,8 +429,8 @@ template <int LA, int LB, int LC, int LD, typename Real>
 hipError_t eri_launch_coop(const EriArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   if (a.nquartets > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((eri_coop_kernel<LA, LB, LC, LD, Real>), grid_1d(a.nquartets, 64),
-                     dim3(64), 0, s, a);
+  constexpr int NT = EriWave<LA, LB, LC, LD, Real>::NT;
+  hipLaunchKernelGGL((eri_coop_kernel<LA, LB, LC, LD, Real>), grid_1d(a.nquartets, NT), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 
