@@ -292,7 +292,14 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) atomicAdd(&gsh[3 * at[i] + k], g[3 * i + k]);
+        for (int k = 0; k < 3; ++k) {
+          // an atom index at or past natom (a caller's wrong count) goes
+          // straight to the global gradient instead of outside gsh
+          if (at[i] < args.natom)
+            atomicAdd(&gsh[3 * at[i] + k], g[3 * i + k]);
+          else
+            atomicAdd(&args.grad[3 * at[i] + k], g[3 * i + k]);
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n3; i += blockDim.x)
@@ -547,6 +554,8 @@ void grad2e_wave_kernel(Grad2eArgs args) {
     sh.hk[k] = ((t * RB + u) * RB + v) | (it << 10) | (iu << 17) | (iv << 24);
   }
   // n / d for 0 <= n < 2^11, 1 <= d <= 2^9 by one multiply: inv = ceil(2^20 / d)
+  // (exact over that range; n = ket primitive x ket pair of a chunk, d = ket pairs)
+  static_assert(KC * NCDC < 2048 && NCDC <= 512, "div_small range");
   auto div_small = [](int n, unsigned inv) -> int { return (int)(((unsigned)n * inv) >> 20); };
 
   Real acc[9];  // dA xyz, dP xyz, dC xyz
