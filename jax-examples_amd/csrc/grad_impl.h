@@ -427,14 +427,14 @@ struct GradWave {
   // lanes' serial work per ket primitive) are built for KE ket primitives
   // at once.  With KC = 1, KE is the largest count (at most GMAXP) that keeps
   // the work-groups per CU (160 KB of LDS in 512-byte granules; the estimate
-  // adds 1 KB for alignment padding) at min(that of KE = 1, 8), except for
-  // (pp) kets, where batching measured slower in every class ((pp|pp),
-  // (ds|pp), (dp|pp), (dd|pp)) and KE = 1.
+  // adds 1 KB for alignment padding) at min(that of KE = 1, 8); for (pp)
+  // kets at that of KE = 1 exactly (measured: giving up occupancy for
+  // batching lost in every (pp)-ket class, keeping it gains up to 5 %).
   static constexpr int occ(int b) { return 163840 / ((b + 1024 + 511) / 512 * 512); }
   static constexpr int pick_ke() {
     if (KC > 1) return KC;
-    if (LC == 1 && LD == 1) return 1;
-    const int want = occ(lds(NCDC, 1, 1)) < 8 ? occ(lds(NCDC, 1, 1)) : 8;
+    const int occ1 = occ(lds(NCDC, 1, 1));
+    const int want = LC == 1 && LD == 1 ? occ1 : (occ1 < 8 ? occ1 : 8);
     int ke = GMAXP;
     while (ke > 1 && occ(lds(NCDC, 1, ke)) < want) --ke;
     return ke;
