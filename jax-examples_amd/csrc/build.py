@@ -35,6 +35,10 @@ ERI_CLASSES = [
 # (the rest have only the one-wave-per-quartet form: their G blocks and
 # Hermite tables do not fit a thread's registers)
 GRAD_PER_THREAD = {"0000", "1000", "1010", "1100", "1110", "2000", "2010", "2100"}
+# classes whose one-thread-per-quartet kernel fits the registers in FP32 only
+# (measured faster than the wave form on (H2O)_16: (ds|ds) 7.4 vs 40.9 ms,
+# (dd|ss) 6.2 vs 9.2 ms; (ds|pp) keeps arrays on the stack and loses)
+GRAD_PER_THREAD_F32 = GRAD_PER_THREAD | {"2020", "2200"}
 STATIC_SOURCES = ["capi.hip", "oneint.hip", "jk.hip", "grad.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -84,7 +88,7 @@ def _generate():
                 + "}\n"
                 f"hipError_t grad2e_launcht32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float>(a, s);\n"
-                   if tag in GRAD_PER_THREAD else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
+                   if tag in GRAD_PER_THREAD_F32 else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
                 + "}\n}  // namespace qcmd\n")
         if not os.path.exists(path) or open(path).read() != text:
             with open(path, "w") as fh:

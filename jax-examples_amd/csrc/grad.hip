@@ -261,7 +261,9 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   // that has that kernel (ket (ss) up to (dp|ss), ket (ps) up to (ds|ps)),
   // one wave with the bra side contracted first for the rest
   const bool has_thread = (ccd == 0 && cab <= 4) || (ccd == 1 && cab <= 3);
-  const int var = in->variant != 0 ? in->variant : (has_thread ? 1 : 2);
+  // in FP32 the thread kernel also fits (ds|ds) and (dd|ss)
+  const bool has_thread32 = has_thread || (cab == 3 && ccd == 3) || (cab == 5 && ccd == 0);
+  const int var = in->variant != 0 ? in->variant : ((in->fp32 ? has_thread32 : has_thread) ? 1 : 2);
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
 #define QC_GC(A, B, C, D)                                                                     \
