@@ -22,9 +22,9 @@ from jax_qc.integrals import engine as eng
 
 # (cab, ccd) -> variant, pair classes ss=0 ps=1 pp=2 ds=3 dp=4 dd=5.
 # Measured on MI355X, (H2O)_8 even-tempered, FP64, per launch (one thread vs
-# one wave per quartet): per-thread wins for (dp|pp) 14.9 vs 124 ms, (dp|ds)
-# 4.3 vs 86, (dd|ps) 3.1 vs 22, (dd|ds) 6.3 vs 16.7; the wave kernel wins for
-# (dp|dp) 18.8 vs 55, (dd|pp) 10.3 vs 68, (dd|dp) 11.4 vs 119, (dd|dd) 3.7
+# one wave per quartet): per-thread wins for (dp|pp) 14.9 vs 78 ms, (dp|ds)
+# 4.3 vs 58, (dd|ps) 3.1 vs 13.8, (dd|ds) 6.3 vs 11.3; the wave kernel wins
+# for (dp|dp) 14.7 vs 55, (dd|pp) 7.2 vs 68, (dd|dp) 7.5 vs 124, (dd|dd) 1.5
 # vs 328.  Lower classes: per-thread (by 5-40x).
 DEFAULT_VARIANTS: Dict[Tuple[int, int], int] = {
     (4, 2): 1, (4, 3): 1, (5, 1): 1, (5, 3): 1,
