@@ -71,7 +71,10 @@ constexpr int GMAXP = 9;  // ket primitive pairs per batch, at most
 // integrals h0_tuv (t+u+v <= la+lb+1) and, per direction, hC_tuv with the
 // ket's C-derivative coefficients (t+u+v <= la+lb); each bra pair ab then
 // takes its A-, B- and undifferentiated E against them.
-template <int LA, int LB, int LC, int LD, typename Real>
+// Ket Cartesian pairs cd in [CD0, CD1) only (one launch per range adds its
+// share of the gradient): a range keeps the live registers of the FP64
+// (pp|ps) and (ds|ps) kernels within the register file.
+template <int LA, int LB, int LC, int LD, typename Real, int CD0, int CD1>
 __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, int64_t gid, double (&g)[12],
                                                       int (&at)[4]) {
   constexpr int LAB = LA + LB, LCD = LC + LD, L1 = LAB + LCD + 1;
@@ -95,27 +98,28 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
   const int64_t oc = args.sp_aoc[2 * kp], od = args.sp_aoc[2 * kp + 1];
   const int64_t nc = args.ncart;
 
+  // G over the launch's cd range; with density screening, max |G| over the
+  // whole block (every range's launch takes the same decision)
+  const bool screen = args.tau_density > 0.0;
   Real gam[NAB][NCD];
+  double gmax = 0.0;
 #pragma unroll
   for (int ab = 0; ab < NAB; ++ab)
 #pragma unroll
     for (int cd = 0; cd < NCD; ++cd) {
+      const bool mine = cd >= CD0 && cd < CD1;
+      if (!mine && !screen) continue;
       const int64_t mu = oa + ab / NB, nu = ob + ab % NB, la = oc + cd / ND, si = od + cd % ND;
       double k = args.dk0[mu * nc + la] * args.dk0[nu * nc + si] +
                  args.dk0[mu * nc + si] * args.dk0[nu * nc + la];
       if (args.nk > 1)
         k += args.dk1[mu * nc + la] * args.dk1[nu * nc + si] +
              args.dk1[mu * nc + si] * args.dk1[nu * nc + la];
-      gam[ab][cd] = (Real)(args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k);
+      const Real gv = (Real)(args.dj[mu * nc + nu] * args.dj[la * nc + si] - args.cx * k);
+      if (mine) gam[ab][cd] = gv;
+      gmax = fmax(gmax, fabs((double)gv));
     }
-  if (args.tau_density > 0.0) {
-    double gmax = 0.0;
-#pragma unroll
-    for (int ab = 0; ab < NAB; ++ab)
-#pragma unroll
-      for (int cd = 0; cd < NCD; ++cd) gmax = fmax(gmax, fabs((double)gam[ab][cd]));
-    if (gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return false;
-  }
+  if (screen && gmax * args.schwarz[bp] * args.schwarz[kp] < args.tau_density) return false;
 
   Real acc[9];
 #pragma unroll
@@ -175,6 +179,7 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
       for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
+          if (c * ND + d < CD0 || c * ND + d >= CD1) continue;
           const int kx = CX[LC][c] + 0, ky = CY[LC][c], kz = CZ[LC][c];
           const int jx = CX[LD][d], jy = CY[LD][d], jz = CZ[LD][d];
           Real h0[NH1], hC[3][NH0];
@@ -274,7 +279,7 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
 // and work-group instead of twelve per quartet)
 constexpr int GT_LDS_ATOMS = 256;
 
-template <int LA, int LB, int LC, int LD, typename Real>
+template <int LA, int LB, int LC, int LD, typename Real, int CD0, int CD1>
 __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
   __shared__ double gsh[3 * GT_LDS_ATOMS];
   const bool staged = args.natom > 0 && args.natom <= GT_LDS_ATOMS;  // uniform
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
   const int64_t gid = flat_block() * blockDim.x + threadIdx.x;
   double g[12];
   int at[4];
-  const bool on = gid < args.nquartets && grad2e_thread_quartet<LA, LB, LC, LD, Real>(args, gid, g, at);
+  const bool on = gid < args.nquartets && grad2e_thread_quartet<LA, LB, LC, LD, Real, CD0, CD1>(args, gid, g, at);
   if (staged) {
     if (on) {
 #pragma unroll
@@ -316,7 +321,18 @@ template <int LA, int LB, int LC, int LD, typename Real = double>
 hipError_t grad2e_launch_thread(const Grad2eArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   const dim3 grid = grid_1d((a.nquartets + 127) / 128, 128);
-  hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real>), grid, dim3(128), 0, s, a);
+  constexpr int NCD = ncart(LC) * ncart(LD);
+  // FP64 (pp|ps) and (ds|ps): one launch per ket Cartesian pair (in one
+  // launch they need about 800 registers and spill to scratch)
+  constexpr bool split = sizeof(Real) == 8 && LC == 1 && LD == 0 && LA + LB == 2;
+  if constexpr (split) {
+    static_assert(NCD == 3, "one launch per (p|s) Cartesian pair");
+    hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real, 0, 1>), grid, dim3(128), 0, s, a);
+    hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real, 1, 2>), grid, dim3(128), 0, s, a);
+    hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real, 2, 3>), grid, dim3(128), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real, 0, NCD>), grid, dim3(128), 0, s, a);
+  }
   return hipGetLastError();
 }
 
