@@ -34,11 +34,12 @@ ERI_CLASSES = [
 # derivative-ERI classes that also get the one-thread-per-quartet kernel
 # (the rest have only the one-wave-per-quartet form: their G blocks and
 # Hermite tables do not fit a thread's registers)
-GRAD_PER_THREAD = {"0000", "1000", "1010", "1100", "1110", "2000", "2010", "2100"}
+GRAD_PER_THREAD = {"0000", "1000", "1010", "1100", "1110", "2000", "2010", "2100", "2020"}
 # classes whose one-thread-per-quartet kernel fits the registers in FP32 only
-# (measured faster than the wave form on (H2O)_16: (ds|ds) 7.4 vs 40.9 ms,
-# (dd|ss) 6.2 vs 9.2 ms; (ds|pp) keeps arrays on the stack and loses)
-GRAD_PER_THREAD_F32 = GRAD_PER_THREAD | {"2020", "2200"}
+# (measured faster than the wave form on (H2O)_16: (dd|ss) 6.2 vs 9.2 ms;
+# (ds|pp) keeps arrays on the stack, (dp|ps) and (dp|ds) hold one wave per
+# SIMD at ~400 registers, and those lose: 760 vs 154 ms for (dp|ps))
+GRAD_PER_THREAD_F32 = GRAD_PER_THREAD | {"2200"}
 STATIC_SOURCES = ["capi.hip", "oneint.hip", "jk.hip", "grad.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -257,12 +257,13 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   a.boys64 = boys_table64();
   if (!a.boys64) return set_error("qc_grad2e: qc_init not called on the current device");
   if (in->variant < 0 || in->variant > 2) return set_error("qc_grad2e: variant %d", in->variant);
-  // default (measured on (H2O)_8): one thread per quartet for every class
-  // that has that kernel (ket (ss) up to (dp|ss), ket (ps) up to (ds|ps)),
-  // one wave with the bra side contracted first for the rest
-  const bool has_thread = (ccd == 0 && cab <= 4) || (ccd == 1 && cab <= 3);
-  // in FP32 the thread kernel also fits (ds|ds) and (dd|ss)
-  const bool has_thread32 = has_thread || (cab == 3 && ccd == 3) || (cab == 5 && ccd == 0);
+  // default (measured on (H2O)_8 and (H2O)_16): one thread per quartet for
+  // every class that has that kernel (ket (ss) up to (dp|ss), ket (ps) up to
+  // (ds|ps), and (ds|ds)), one wave with the bra side contracted first for
+  // the rest
+  const bool has_thread = (ccd == 0 && cab <= 4) || (ccd == 1 && cab <= 3) || (cab == 3 && ccd == 3);
+  // in FP32 the thread kernel also fits (dd|ss)
+  const bool has_thread32 = has_thread || (cab == 5 && ccd == 0);
   const int var = in->variant != 0 ? in->variant : ((in->fp32 ? has_thread32 : has_thread) ? 1 : 2);
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;

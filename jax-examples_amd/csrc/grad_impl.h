@@ -317,22 +317,34 @@ __global__ __launch_bounds__(128) void grad2e_thread_kernel(Grad2eArgs args) {
   }
 }
 
+// Ket Cartesian pairs per launch of the thread-per-quartet kernel: one where
+// all of them in one launch would not fit the registers well (FP64 (pp|ps)
+// and (ds|ps) need about 800 and spill: 1.01 -> 0.17 s and 0.17 -> 0.10 s on
+// (H2O)_16 per pair; (ds|ds) FP64 fits only per pair, 0.03 s against 0.35 s
+// for the wave form; FP32 (pp|ps) 52 -> 45 ms), all of them otherwise.
+template <int LA, int LB, int LC, int LD, typename Real>
+constexpr int thread_cd_chunk() {
+  constexpr bool f64 = sizeof(Real) == 8;
+  if (LC == 1 && LD == 0 && LA + LB == 2) return 1;
+  if (f64 && LA == 2 && LB == 0 && LC == 2 && LD == 0) return 1;
+  return ncart(LC) * ncart(LD);
+}
+
+template <int LA, int LB, int LC, int LD, typename Real, int CD0, int CH, int NCD>
+void grad2e_launch_thread_chunks(dim3 grid, const Grad2eArgs& a, hipStream_t s) {
+  if constexpr (CD0 < NCD) {
+    constexpr int CD1 = CD0 + CH < NCD ? CD0 + CH : NCD;
+    hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real, CD0, CD1>), grid, dim3(128), 0, s, a);
+    grad2e_launch_thread_chunks<LA, LB, LC, LD, Real, CD1, CH, NCD>(grid, a, s);
+  }
+}
+
 template <int LA, int LB, int LC, int LD, typename Real = double>
 hipError_t grad2e_launch_thread(const Grad2eArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   const dim3 grid = grid_1d((a.nquartets + 127) / 128, 128);
   constexpr int NCD = ncart(LC) * ncart(LD);
-  // FP64 (pp|ps) and (ds|ps): one launch per ket Cartesian pair (in one
-  // launch they need about 800 registers and spill to scratch)
-  constexpr bool split = sizeof(Real) == 8 && LC == 1 && LD == 0 && LA + LB == 2;
-  if constexpr (split) {
-    static_assert(NCD == 3, "one launch per (p|s) Cartesian pair");
-    hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real, 0, 1>), grid, dim3(128), 0, s, a);
-    hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real, 1, 2>), grid, dim3(128), 0, s, a);
-    hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real, 2, 3>), grid, dim3(128), 0, s, a);
-  } else {
-    hipLaunchKernelGGL((grad2e_thread_kernel<LA, LB, LC, LD, Real, 0, NCD>), grid, dim3(128), 0, s, a);
-  }
+  grad2e_launch_thread_chunks<LA, LB, LC, LD, Real, 0, thread_cd_chunk<LA, LB, LC, LD, Real>(), NCD>(grid, a, s);
   return hipGetLastError();
 }
 

@@ -43,7 +43,7 @@ def grad_variant_for(cab: int, ccd: int) -> int:
 
 
 # classes with a one-thread-per-quartet kernel (as built by csrc/build.py)
-THREAD_CAPABLE = {(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1), (4, 0)}
+THREAD_CAPABLE = {(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1), (4, 0), (3, 3)}
 
 
 def energy_weighted_density(C, orbital_energies, n_occ: int, factor: float = 2.0):
