@@ -69,7 +69,11 @@ __device__ inline void locate(const EriArgs& a, int64_t gid, int& bp, int& kp) {
   kp = a.ket_list[a.kstart[lo] + (int)(gid - a.offsets[lo])];
 }
 
-template <int LA, int LB, int LC, int LD, typename Real>
+// Ket Cartesian pairs cd in [CD0, CD1) only: for kets up to (pp) the ket's
+// spherical transform is the identity, so a launch per range computes and
+// writes exactly its own integrals (not used in Schwarz mode, which needs the
+// whole diagonal block).
+template <int LA, int LB, int LC, int LD, typename Real, int CD0, int CD1>
 __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
   constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -123,6 +127,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
       for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
+          if (c * ND + d < CD0 || c * ND + d >= CD1) continue;
           const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
           const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
           Real h[LAB + 1][LAB + 1][LAB + 1];
@@ -230,6 +235,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
     for (int sb = 0; sb < SB; ++sb)
 #pragma unroll
       for (int k = 0; k < SC * SD; ++k) {
+        if (k < CD0 || k >= CD1) continue;  // a range only with identity ket transforms
         Real v = (Real)0;
 #pragma unroll
         for (int b = 0; b < NB; ++b)
@@ -246,6 +252,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
     for (int sb = 0; sb < SB; ++sb)
 #pragma unroll
       for (int k = 0; k < SC * SD; ++k) {
+        if (k < CD0 || k >= CD1) continue;
         Real v = (Real)0;
 #pragma unroll
         for (int a = 0; a < NA; ++a)
@@ -270,12 +277,38 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
 
 // Launch one class; instantiated once per (class, precision) in its own
 // translation unit (generated by build.py) so the classes compile in parallel.
+// Ket Cartesian pairs per launch: one for the FP64 (dp|pp) and (dd|ps)
+// kernels, which spill to scratch as single launches; all of them otherwise
+// (and always in Schwarz mode).
+template <int LA, int LB, int LC, int LD, typename Real>
+constexpr int eri_cd_chunk() {
+  constexpr bool f64 = sizeof(Real) == 8;
+  if (f64 && LA == 2 && LB == 1 && LC == 1 && LD == 1) return 1;
+  if (f64 && LA == 2 && LB == 2 && LC == 1 && LD == 0) return 1;
+  return ncart(LC) * ncart(LD);
+}
+
+template <int LA, int LB, int LC, int LD, typename Real, int CD0, int CH, int NCD>
+void eri_launch_class_chunks(dim3 grid, int block, const EriArgs& a, hipStream_t s) {
+  if constexpr (CD0 < NCD) {
+    constexpr int CD1 = CD0 + CH < NCD ? CD0 + CH : NCD;
+    hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, Real, CD0, CD1>), grid, dim3(block), 0, s, a);
+    eri_launch_class_chunks<LA, LB, LC, LD, Real, CD1, CH, NCD>(grid, block, a, s);
+  }
+}
+
 template <int LA, int LB, int LC, int LD, typename Real>
 hipError_t eri_launch_class(const EriArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   const int block = a.block == 64 ? 64 : 128;
   const dim3 grid = grid_1d((a.nquartets + block - 1) / block, block);
-  hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, Real>), grid, dim3(block), 0, s, a);
+  constexpr int NCD = ncart(LC) * ncart(LD);
+  constexpr int CH = eri_cd_chunk<LA, LB, LC, LD, Real>();
+  static_assert(CH == NCD || (LC <= 1 && LD <= 1), "ket ranges need identity ket transforms");
+  if (CH < NCD && a.mode != ERI_SCHWARZ)
+    eri_launch_class_chunks<LA, LB, LC, LD, Real, 0, CH, NCD>(grid, block, a, s);
+  else
+    hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, Real, 0, NCD>), grid, dim3(block), 0, s, a);
   return hipGetLastError();
 }
 
