@@ -72,9 +72,12 @@ __device__ inline void locate(const EriArgs& a, int64_t gid, int& bp, int& kp) {
 // Ket Cartesian pairs cd in [CD0, CD1) only: for kets up to (pp) the ket's
 // spherical transform is the identity, so a launch per range computes and
 // writes exactly its own integrals (not used in Schwarz mode, which needs the
-// whole diagonal block).
-template <int LA, int LB, int LC, int LD, typename Real, int CD0, int CD1>
+// whole diagonal block).  Likewise bra Cartesians b in [B0, B1) only, for a
+// p (or s) shell b.
+template <int LA, int LB, int LC, int LD, typename Real, int CD0, int CD1, int B0 = 0,
+          int B1 = ncart(LB)>
 __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
+  static_assert((B0 == 0 && B1 == ncart(LB)) || LB <= 1, "bra ranges need an identity b transform");
   constexpr int LAB = LA + LB, LCD = LC + LD, L = LAB + LCD;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   const int64_t gid = flat_block() * blockDim.x + threadIdx.x;
@@ -161,6 +164,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
           for (int a = 0; a < NA; ++a)
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
+              if (b < B0 || b >= B1) continue;
               const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
               const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
               Real val = (Real)0;
@@ -209,6 +213,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
     for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int sd = 0; sd < SD; ++sd) {
+        if (i % NB < B0 || i % NB >= B1) continue;
         Real v = (Real)0;
 #pragma unroll
         for (int d = 0; d < ND; ++d)
@@ -222,6 +227,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
     for (int sc = 0; sc < SC; ++sc)
 #pragma unroll
       for (int sd = 0; sd < SD; ++sd) {
+        if (i % NB < B0 || i % NB >= B1) continue;
         Real v = (Real)0;
 #pragma unroll
         for (int c = 0; c < NC; ++c)
@@ -236,6 +242,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
 #pragma unroll
       for (int k = 0; k < SC * SD; ++k) {
         if (k < CD0 || k >= CD1) continue;  // a range only with identity ket transforms
+        if (sb < B0 || sb >= B1) continue;  // likewise for b
         Real v = (Real)0;
 #pragma unroll
         for (int b = 0; b < NB; ++b)
@@ -253,6 +260,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
 #pragma unroll
       for (int k = 0; k < SC * SD; ++k) {
         if (k < CD0 || k >= CD1) continue;
+        if (sb < B0 || sb >= B1) continue;
         Real v = (Real)0;
 #pragma unroll
         for (int a = 0; a < NA; ++a)
@@ -299,6 +307,25 @@ void eri_launch_class_chunks(dim3 grid, int block, const EriArgs& a, hipStream_t
   }
 }
 
+// Bra Cartesians b per launch, for a p shell b: one for FP64 (dp|ds), whose
+// d ket rules out ket ranges and which spills to scratch as one launch; all
+// of them otherwise.
+template <int LA, int LB, int LC, int LD, typename Real>
+constexpr int eri_b_chunk() {
+  constexpr bool f64 = sizeof(Real) == 8;
+  if (f64 && LA == 2 && LB == 1 && LC == 2 && LD == 0) return 1;
+  return ncart(LB);
+}
+
+template <int LA, int LB, int LC, int LD, typename Real, int B0, int NB>
+void eri_launch_class_bchunks(dim3 grid, int block, const EriArgs& a, hipStream_t s) {
+  if constexpr (B0 < NB) {
+    hipLaunchKernelGGL((eri_class_kernel<LA, LB, LC, LD, Real, 0, ncart(LC) * ncart(LD), B0, B0 + 1>),
+                       grid, dim3(block), 0, s, a);
+    eri_launch_class_bchunks<LA, LB, LC, LD, Real, B0 + 1, NB>(grid, block, a, s);
+  }
+}
+
 template <int LA, int LB, int LC, int LD, typename Real>
 hipError_t eri_launch_class(const EriArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
@@ -307,6 +334,14 @@ hipError_t eri_launch_class(const EriArgs& a, hipStream_t s) {
   constexpr int NCD = ncart(LC) * ncart(LD);
   constexpr int CH = eri_cd_chunk<LA, LB, LC, LD, Real>();
   static_assert(CH == NCD || (LC <= 1 && LD <= 1), "ket ranges need identity ket transforms");
+  constexpr int BCH = eri_b_chunk<LA, LB, LC, LD, Real>();
+  static_assert(BCH == ncart(LB) || (BCH == 1 && LB <= 1 && CH == NCD), "bra ranges: one b each, p shells");
+  if constexpr (BCH < ncart(LB)) {
+    if (a.mode != ERI_SCHWARZ) {
+      eri_launch_class_bchunks<LA, LB, LC, LD, Real, 0, ncart(LB)>(grid, block, a, s);
+      return hipGetLastError();
+    }
+  }
   if (CH < NCD && a.mode != ERI_SCHWARZ)
     eri_launch_class_chunks<LA, LB, LC, LD, Real, 0, CH, NCD>(grid, block, a, s);
   else
