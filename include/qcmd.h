@@ -116,6 +116,22 @@ int qc_sm_pack(const double* M, int64_t nbf, const uint8_t* fmt, const int64_t* 
 int qc_symv_mixed_tiles(const void* pool, const uint8_t* fmt, const int64_t* off, int64_t nbf,
                         int nrhs, const double* x, void* work, void* stream);
 
+/* Screened Fock build over a mixed pool, for builds from a density
+ * difference (incremental SCF): the same contraction as the reference's
+ * J/K einsums (jax_qc/scf/fock.py:31-38) with the tiles whose
+ * tmax[t] * max(xmax[R], xmax[C]) < tau left out (each left-out tile moves
+ * any element of y by less than 64 tau).  tmax: per-tile max |element| of
+ * the pool (FP32); xmax[B]: max |x| over block B's 64 packed rows and every
+ * right-hand side, from qc_block_maxabs; tau >= 0 (0 still skips zero
+ * tiles).  qc_symv_screened_reduce finishes the build and must get the same
+ * fmt, tmax, xmax and tau as the tile pass. */
+int qc_block_maxabs(const double* x, int64_t nbf, int nrhs, float* xmax, void* stream);
+int qc_symv_screened_tiles(const void* pool, const uint8_t* fmt, const int64_t* off,
+                           const float* tmax, int64_t nbf, int nrhs, const double* x,
+                           const float* xmax, double tau, void* work, void* stream);
+int qc_symv_screened_reduce(int64_t nbf, int nrhs, double* y, void* work, const uint8_t* fmt,
+                            const float* tmax, const float* xmax, double tau, void* stream);
+
 /* x[r][pair(k,l)] = D_r[k][l] * (k == l ? 1 : 2) for r < nrhs. */
 int qc_pack_density(const double* D, int64_t nbf, int nrhs, double* x, void* stream);
 

@@ -174,6 +174,26 @@ __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile
   if (!diag) store_cols<NRHS, T>(col, tile, cg, ro, colbuf);
 }
 
+// Optional screen of a compacted supermatrix's tiles: a tile (R, C) is
+// skipped when tmax[tile] * max(xmax[R], xmax[C]) < tau, with tmax the
+// tile's max |element| and xmax[B] the max |x| over the 64 packed rows of
+// block B and every right-hand side -- its contribution to any element of y
+// is then below 64 tau.  Zero-format tiles are skipped as well, and the
+// screened reduce leaves the skipped tiles' column partials unread, so
+// nothing is written for them.  tmax == nullptr: no screen (zero tiles then
+// write zero column partials for the plain reduce).
+struct TileScreen {
+  const float* tmax;
+  const float* xmax;
+  float tau;
+};
+
+__device__ __forceinline__ bool tile_live(const uint8_t* __restrict__ fmt, const TileScreen& scr,
+                                          int64_t tile, float xa, float xb) {
+  if (fmt[tile] == TILE_ZERO) return false;
+  return scr.tmax[tile] * fmaxf(xa, xb) >= scr.tau;
+}
+
 // One wave per work item (tile row R, a segment of its tiles) of one
 // supermatrix whose tiles start at global tile tile0 and whose packed vector
 // rows start at global row row0 (0 and 0 for a single matrix; many
@@ -189,7 +209,7 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
                                           const double* __restrict__ x, int64_t npad,
                                           double* __restrict__ rowbuf, double* __restrict__ colbuf,
                                           int seg, int64_t item, int64_t gitem, int64_t tile0,
-                                          int64_t row0) {
+                                          int64_t row0, const TileScreen& scr) {
   int64_t lo = 0, hi = T - 1;
   while (lo < hi) {
     int64_t mid = (lo + hi + 1) >> 1;
@@ -210,6 +230,7 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
       xr[h][k] = x[h * npad + (row0 + R) * SM_B + 4 * k + ro];
       rowacc[h][k] = 0.0;
     }
+  const float xmR = (MIXED && scr.tmax) ? scr.xmax[row0 + R] : 0.0f;
 
   for (int64_t C = C0; C < C1; ++C) {
     const int64_t tile = tile0 + R * (R + 1) / 2 + C;
@@ -220,6 +241,7 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
       symv_tile<NRHS, NT, double>(tp, tile, diag, xr, rowacc, xcb, npad, cg, ro, colbuf);
     } else {
       const uint8_t f = fmt[tile];
+      if (scr.tmax && !tile_live(fmt, scr, tile, xmR, scr.xmax[row0 + C])) continue;
       const char* base = reinterpret_cast<const char*>(Mv) + off[tile];
       if (f == TILE_F64) {
         symv_tile<NRHS, NT, double>(reinterpret_cast<const double*>(base), tile, diag, xr, rowacc,
@@ -263,9 +285,10 @@ __global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_tiles_kernel(const vo
                                                         const int64_t* __restrict__ off,
                                                         const double* __restrict__ x, int64_t npad,
                                                         double* __restrict__ rowbuf,
-                                                        double* __restrict__ colbuf, int seg) {
+                                                        double* __restrict__ colbuf, int seg,
+                                                        TileScreen scr) {
   const int64_t item = blockIdx.x;
-  symv_item<NRHS, MIXED, NT>(Mv, T, fmt, off, x, npad, rowbuf, colbuf, seg, item, item, 0, 0);
+  symv_item<NRHS, MIXED, NT>(Mv, T, fmt, off, x, npad, rowbuf, colbuf, seg, item, item, 0, 0, scr);
 }
 
 // Last index i in [0, n) with pre[i] <= v (pre ascending, pre[0] = 0).
@@ -294,10 +317,11 @@ __global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_batch_kernel(
     const int64_t* __restrict__ row0) {
   const int64_t total = item0[nmol];
   const int64_t stride = (int64_t)gridDim.x * gridDim.y;
+  const TileScreen none = {nullptr, nullptr, 0.0f};
   for (int64_t g = flat_block(); g < total; g += stride) {
     const int m = find_seg(item0, nmol, g);
     symv_item<NRHS, MIXED, NT>(Mv, molT[m], fmt, off, x, npad, rowbuf, colbuf, seg, g - item0[m],
-                               g, tile0[m], row0[m]);
+                               g, tile0[m], row0[m], none);
   }
 }
 
@@ -336,15 +360,19 @@ __global__ __launch_bounds__(256) void tile_pack_kernel(const double* __restrict
 __global__ __launch_bounds__(256) void symv_reduce_kernel(int nrhs, int64_t T, int64_t npad,
                                                           const double* __restrict__ rowbuf,
                                                           const double* __restrict__ colbuf,
-                                                          double* __restrict__ y, int seg) {
+                                                          double* __restrict__ y, int seg,
+                                                          const uint8_t* __restrict__ fmt,
+                                                          TileScreen scr) {
   const int64_t R = blockIdx.x;
   const int h = blockIdx.y;
   const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
   double acc = 0.0;
   const int64_t i0 = item_off(R, seg), i1 = item_off(R + 1, seg);
   for (int64_t it = i0 + w; it < i1; it += 4) acc += rowbuf[(it * nrhs + h) * SM_B + r];
+  const float xmR = scr.tmax ? scr.xmax[R] : 0.0f;
   for (int64_t Rp = R + 1 + w; Rp < T; Rp += 4) {
     const int64_t tile = Rp * (Rp + 1) / 2 + R;
+    if (scr.tmax && !tile_live(fmt, scr, tile, scr.xmax[Rp], xmR)) continue;
     acc += colbuf[(tile * nrhs + h) * SM_B + r];
   }
   __shared__ double part[4][64];
@@ -427,6 +455,16 @@ __global__ void pack_density_kernel(const double* __restrict__ D, int64_t nbf, i
   x[h * npad + p] = v;
 }
 
+// xmax[B] = max over the right-hand sides and the 64 rows of block B of |x|.
+__global__ __launch_bounds__(64) void block_maxabs_kernel(const double* __restrict__ x, int nrhs,
+                                                         int64_t npad, float* __restrict__ xmax) {
+  const int64_t B = blockIdx.x;
+  double m = 0.0;
+  for (int h = 0; h < nrhs; ++h) m = fmax(m, fabs(x[h * npad + B * SM_B + threadIdx.x]));
+  for (int w = 32; w > 0; w >>= 1) m = fmax(m, __shfl_xor(m, w));
+  if (threadIdx.x == 0) xmax[B] = (float)m;
+}
+
 __global__ void unpack_fock_kernel(const double* __restrict__ H, const double* __restrict__ y,
                                    double scale, int64_t nbf, int64_t npad,
                                    double* __restrict__ F) {
@@ -502,7 +540,8 @@ int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs) {
 
 static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const double* x,
                       double* y, void* work, void* stream, const uint8_t* fmt = nullptr,
-                      const int64_t* off = nullptr) {
+                      const int64_t* off = nullptr,
+                      TileScreen scr = TileScreen{nullptr, nullptr, 0.0f}) {
   if (nrhs < 1 || nrhs > 4) return set_error("qc_symv: nrhs %d outside [1, 4]", nrhs);
   if (nbf <= 0) return set_error("qc_symv: nbf %lld", (long long)nbf);
   if (!M || !x || !work || (phase != 1 && !y)) return set_error("qc_symv: null pointer");
@@ -516,7 +555,7 @@ static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const d
     switch (nrhs) {
 #define QC_L(N, MX, NT)                                                                     \
   hipLaunchKernelGGL((symv_tiles_kernel<N, MX, NT>), dim3((unsigned)items), dim3(64), 0, s, \
-                     (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf, seg)
+                     (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf, seg, scr)
 #define QC_S(N)                     \
   case N:                           \
     if (fmt) {                      \
@@ -539,7 +578,7 @@ static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const d
   }
   if (phase & 2) {
     hipLaunchKernelGGL(symv_reduce_kernel, dim3((unsigned)T, nrhs), dim3(256), 0, s, nrhs, T,
-                       npad, rowbuf, colbuf, y, seg);
+                       npad, rowbuf, colbuf, y, seg, fmt, scr);
     if (check(hipGetLastError(), "qc_symv reduce")) return 1;
   }
   return 0;
@@ -559,6 +598,38 @@ int qc_symv_mixed_tiles(const void* pool, const uint8_t* fmt, const int64_t* off
                         int nrhs, const double* x, void* work, void* stream) {
   if (!fmt || !off) return set_error("qc_symv_mixed_tiles: null tile directory");
   return symv_phase(1, (const double*)pool, nbf, nrhs, x, nullptr, work, stream, fmt, off);
+}
+
+int qc_block_maxabs(const double* x, int64_t nbf, int nrhs, float* xmax, void* stream) {
+  if (nrhs < 1 || nrhs > 4) return set_error("qc_block_maxabs: nrhs %d outside [1, 4]", nrhs);
+  if (nbf <= 0 || !x || !xmax) return set_error("qc_block_maxabs: bad arguments");
+  const int64_t T = sm_tiles_per_dim(nbf), npad = T * SM_B;
+  hipLaunchKernelGGL(block_maxabs_kernel, dim3((unsigned)T), dim3(64), 0, (hipStream_t)stream, x,
+                     nrhs, npad, xmax);
+  return check(hipGetLastError(), "qc_block_maxabs");
+}
+
+static int screen_args(const char* who, const uint8_t* fmt, const float* tmax,
+                       const float* xmax, double tau) {
+  if (!fmt || !tmax || !xmax) return set_error("%s: null tile directory or maxima", who);
+  if (!(tau >= 0.0)) return set_error("%s: tau %g must be >= 0", who, tau);
+  return 0;
+}
+
+int qc_symv_screened_tiles(const void* pool, const uint8_t* fmt, const int64_t* off,
+                           const float* tmax, int64_t nbf, int nrhs, const double* x,
+                           const float* xmax, double tau, void* work, void* stream) {
+  if (!off) return set_error("qc_symv_screened_tiles: null tile offsets");
+  if (screen_args("qc_symv_screened_tiles", fmt, tmax, xmax, tau)) return 1;
+  return symv_phase(1, (const double*)pool, nbf, nrhs, x, nullptr, work, stream, fmt, off,
+                    TileScreen{tmax, xmax, (float)tau});
+}
+
+int qc_symv_screened_reduce(int64_t nbf, int nrhs, double* y, void* work, const uint8_t* fmt,
+                            const float* tmax, const float* xmax, double tau, void* stream) {
+  if (screen_args("qc_symv_screened_reduce", fmt, tmax, xmax, tau)) return 1;
+  return symv_phase(2, (const double*)work, nbf, nrhs, (const double*)work, y, work, stream, fmt,
+                    nullptr, TileScreen{tmax, xmax, (float)tau});
 }
 
 int qc_tiles_maxabs(const double* M, int64_t ntiles, double* tmax, void* stream) {

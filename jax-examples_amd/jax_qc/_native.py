@@ -66,6 +66,11 @@ _SIGS = {
     "qc_symv_mixed_tiles": [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p,
                             c_void_p],
     "qc_symv_config": [c_int, c_int],
+    "qc_block_maxabs": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
+    "qc_symv_screened_tiles": [c_void_p, c_void_p, c_void_p, c_void_p, c_i64, c_int, c_void_p,
+                               c_void_p, c_double, c_void_p, c_void_p],
+    "qc_symv_screened_reduce": [c_i64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_double, c_void_p],
     "qc_pack_density": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
     "qc_unpack_fock": [c_void_p, c_void_p, c_double, c_i64, c_int, c_void_p, c_void_p],
     "qc_dense_jk": [c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p],

@@ -36,21 +36,46 @@ class OptResult(NamedTuple):
 
 
 def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12, tau32=None,
-                        tile_f32=None, grad_tau=1e-13, grad_tau_density=0.0, grad_tau32=None):
+                        tile_f32=None, grad_tau=1e-13, grad_tau_density=0.0, grad_tau32=None,
+                        prim_eps=None, fock_tau=None, fock_full_every=16, timings=None):
+    """Energy and gradient at ``coords``.  Mixed-path options: tau32 (FP32
+    ERI sums below that Schwarz bound), tile_f32 (FP32 supermatrix tiles),
+    prim_eps (primitive-pair screening of the ERIs and derivative ERIs),
+    fock_tau (incremental Fock builds with tile screening), grad_tau_density
+    and grad_tau32 (gradient screening / FP32 sums).  ``timings`` (a dict)
+    receives the wall time of each stage, the device synchronised between
+    them."""
+    import time
+
+    def mark(name, t0):
+        if timings is not None:
+            torch.cuda.synchronize()
+            timings[name] = time.perf_counter() - t0
+        return time.perf_counter()
+
+    t = time.perf_counter()
     m = displaced(mol, coords)
     bs = build_basis_set(m, basis_name)
-    ints = compute_integrals(m, bs, eri=eri, tau=tau, tau32=tau32)
-    if tile_f32 is not None and eri == "store":
+    ints = compute_integrals(m, bs, eri=eri, tau=tau, tau32=tau32, prim_eps=prim_eps)
+    if eri == "store":
         if config.method.lower() == "uhf":
             ints.ERI.ensure_K()
-            ints.ERI.use_compact(tau_f32=tile_f32)
         else:
             ints.ERI.ensure_P()
             ints.ERI.release_J()
+        if tile_f32 is not None:
             ints.ERI.use_compact(tau_f32=tile_f32)
+        if fock_tau is not None:
+            ints.ERI.set_incremental(fock_tau, fock_full_every)
+    t = mark("integrals", t)
     res = run_scf(m, ints, config)
+    t = mark("scf", t)
     g = scf_gradient(m, bs, res, grad_tau, pair_table=getattr(ints.ERI, "pt", None),
                      tau_density=grad_tau_density, tau32=grad_tau32)
+    t = mark("gradient", t)
+    if timings is not None and hasattr(ints.ERI, "inc") and ints.ERI.inc is not None:
+        timings["fock_full_builds"] = ints.ERI.inc.n_full
+        timings["fock_incremental_builds"] = ints.ERI.inc.n_inc
     # drop the integral store (tens of GB at 480 functions) before the
     # caller's next evaluation
     res = dataclasses.replace(res, ERI=None)

@@ -98,17 +98,18 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
     grad = torch.zeros(nat_ * 3, dtype=torch.float64, device=dev)
     s = nat.stream_ptr()
     if "1e" in parts:
+        pt1 = getattr(pt, "unscreened", pt)  # every primitive for the 1e terms
         xyz = torch.as_tensor(np.ascontiguousarray(mol.coords, dtype=np.float64), device=dev)
         zz = torch.as_tensor(np.asarray(mol.atomic_numbers, dtype=np.float64), device=dev)
         for c in range(6):
-            lst = pt.class_pairs[c]
+            lst = pt1.class_pairs[c]
             if len(lst) == 0:
                 continue
             ld = torch.from_numpy(lst).to(dev)
-            nat.call("qc_grad1e", c, nat.ptr(pt.pairs64), nat.ptr(pt.sp_prim_off),
-                     nat.ptr(pt.sp_nprim), nat.ptr(pt.sp_aoc), nat.ptr(pt.sp_atom),
-                     nat.ptr(pt.sp_diag), nat.ptr(ld), len(lst), nat_, nat.ptr(xyz), nat.ptr(zz),
-                     pt.ncart, nat.ptr(Djc), nat.ptr(Wc), nat.ptr(grad), s)
+            nat.call("qc_grad1e", c, nat.ptr(pt1.pairs64), nat.ptr(pt1.sp_prim_off),
+                     nat.ptr(pt1.sp_nprim), nat.ptr(pt1.sp_aoc), nat.ptr(pt1.sp_atom),
+                     nat.ptr(pt1.sp_diag), nat.ptr(ld), len(lst), nat_, nat.ptr(xyz), nat.ptr(zz),
+                     pt1.ncart, nat.ptr(Djc), nat.ptr(Wc), nat.ptr(grad), s)
     if "2e" in parts:
         plan = eng.plan_quartets(pt, tau, tau32)
         Qd = torch.from_numpy(pt.Q).to(dev) if tau_density > 0.0 else None

@@ -180,6 +180,66 @@ class PairTable:
         return a
 
 
+class _PrimitivePairView:
+    """Every primitive pair of a PairTable as a one-primitive 'shell pair'
+    (same records, same class), so that ``schwarz`` returns the per-primitive
+    -pair bound sqrt(max_ab (ab|ab)) over its Cartesian components."""
+
+    def __init__(self, pt: PairTable):
+        n = pt.sp_nprim.cpu().numpy().astype(np.int64)
+        owner = np.repeat(np.arange(pt.npairs), n)
+        d = pt.dev
+        self.dev, self.nbf, self.npairs = d, pt.nbf, int(n.sum())
+        self.pairs64, self.pairs32 = pt.pairs64, pt.pairs32
+        self.sp_prim_off = torch.arange(self.npairs, dtype=torch.int32, device=d)
+        self.sp_nprim = torch.ones(self.npairs, dtype=torch.int32, device=d)
+        self.sp_ao = pt.sp_ao[torch.from_numpy(owner).to(d)].contiguous()
+        cls = pt.sp_class[owner]
+        self.class_pairs = {c: np.nonzero(cls == c)[0].astype(np.int32) for c in range(6)}
+        self.owner = owner
+        self.Q = None
+    base_args = PairTable.base_args
+
+
+def screen_primitives(pt: PairTable, eps: float) -> PairTable:
+    """A copy of ``pt`` without the primitive pairs k whose bound
+    Q_k = sqrt(max (ab|ab)_k) satisfies Q_k * Q_max < eps, Q_max the largest
+    primitive-pair bound of the table: by the Cauchy-Schwarz inequality of
+    the Coulomb inner product every primitive quartet left out is below eps
+    in magnitude (times the few-percent spread of the spherical transform).
+    Every shell pair keeps at least its largest primitive pair; the
+    shell-pair bounds (and so the quartet plans) are those of ``pt``.  This
+    is primitive-level screening for the mixed path: the ERIs and derivative
+    integrals of distant pairs of tight primitives, whose Gaussian-product
+    factor exp(-ab/p |AB|^2) vanishes, are never evaluated."""
+    import copy
+
+    if pt.Q is None:
+        schwarz(pt)
+    view = _PrimitivePairView(pt)
+    qp = schwarz(view)
+    keep = qp * float(qp.max()) >= eps
+    owner = view.owner
+    # every shell pair keeps its largest primitive pair
+    best = np.full(pt.npairs, -1.0)
+    np.maximum.at(best, owner, qp)
+    keep |= qp == best[owner]
+    rec = pt.pairs64[torch.from_numpy(np.nonzero(keep)[0]).to(pt.dev)].contiguous()
+    cnt = np.bincount(owner[keep], minlength=pt.npairs).astype(np.int64)
+    off = np.zeros(pt.npairs, dtype=np.int64)
+    off[1:] = np.cumsum(cnt)[:-1]
+    out = copy.copy(pt)
+    out.pairs64 = rec
+    out.pairs32 = rec.float() if pt.pairs32 is not None else None
+    out.sp_prim_off = torch.from_numpy(off.astype(np.int32)).to(pt.dev)
+    out.sp_nprim = torch.from_numpy(cnt.astype(np.int32)).to(pt.dev)
+    out.nprim_pairs = int(cnt.sum())
+    out.prim_eps = float(eps)
+    out.prim_kept = float(keep.mean())
+    out.unscreened = pt  # one-electron integrals and their derivatives use every primitive
+    return out
+
+
 def schwarz(pt: PairTable) -> np.ndarray:
     """Per-shell-pair Q = sqrt(max_ab (ab|ab)) (Cartesian diagonal)."""
     nat.ensure_init(pt.dev)

@@ -27,7 +27,11 @@ from jax_qc.profiling import optional_stage
 
 
 def compute_integrals(mol, basis, timer=None, eri: str = "auto", tau: Optional[float] = None,
-                      tau32: Optional[float] = None) -> IntegralSet:
+                      tau32: Optional[float] = None, prim_eps: Optional[float] = None) -> IntegralSet:
+    """tau: Schwarz threshold; tau32: quartets with a bound below it sum
+    their primitives in FP32; prim_eps: the ERI store leaves out primitive
+    pairs whose Cauchy-Schwarz bound against the largest one is below it
+    (engine.screen_primitives; mixed path only)."""
     dev = nat.device()
     nat.ensure_init(dev)
     if eri == "auto":
@@ -46,7 +50,8 @@ def compute_integrals(mol, basis, timer=None, eri: str = "auto", tau: Optional[f
             if eri == "dense":
                 ERI = compute_eri_tensor(basis, 0.0 if tau is None else tau, tau32, pt)
             elif eri == "store":
-                ERI = ERIStore(basis, 1e-12 if tau is None else tau, tau32, dev, pt)
+                ept = eng.screen_primitives(pt, prim_eps) if prim_eps else pt
+                ERI = ERIStore(basis, 1e-12 if tau is None else tau, tau32, dev, ept)
             else:
                 raise ValueError(f"eri must be 'auto', 'dense' or 'store', got {eri!r}")
         with optional_stage(timer, "nuclear_repulsion"):

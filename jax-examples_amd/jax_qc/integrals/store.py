@@ -42,6 +42,7 @@ class CompactSupermatrix:
     pool: torch.Tensor      # uint8 byte pool
     fmt: torch.Tensor       # uint8 per tile
     off: torch.Tensor       # int64 byte offset per tile
+    tmax: torch.Tensor      # float32 max |element| per tile (tile screening)
     n_zero: int
     n_f32: int
     n_f64: int
@@ -72,7 +73,8 @@ class ERIStore:
         torch.cuda.synchronize(self.dev)
         self.eri_seconds = time.perf_counter() - t0
         self.P = self.K = None
-        self._work, self._x, self._y = {}, {}, {}
+        self._work, self._x, self._y, self._xm = {}, {}, {}, {}
+        self.inc = None
 
     def use_compact(self, tau_f32: float = 1e-4, tau_zero: float = 1e-14):
         """Replace the assembled supermatrices (P for RHF; J and K for UHF)
@@ -89,8 +91,16 @@ class ERIStore:
             self.J = self.compact(self.J, tau_f32, tau_zero)
         return self
 
+    def _assemble_from_J(self, what):
+        if not isinstance(self.J, torch.Tensor):
+            raise RuntimeError(
+                f"cannot assemble {what}: the J supermatrix was "
+                + ("released" if self.J is None else "compacted") +
+                "; assemble P or K before use_compact() / release_J()")
+
     def ensure_P(self):
         if self.P is None:
+            self._assemble_from_J("P")
             self.P = torch.empty_like(self.J)
             nat.call("qc_sm_assemble", 0, nat.ptr(self.J), nat.ptr(self.P), self.nbf,
                      nat.stream_ptr())
@@ -98,6 +108,7 @@ class ERIStore:
 
     def ensure_K(self):
         if self.K is None:
+            self._assemble_from_J("K")
             self.K = torch.empty_like(self.J)
             nat.call("qc_sm_assemble", 1, nat.ptr(self.J), nat.ptr(self.K), self.nbf,
                      nat.stream_ptr())
@@ -127,7 +138,9 @@ class ERIStore:
         od = torch.from_numpy(off).to(self.dev)
         nat.call("qc_sm_pack", nat.ptr(M), self.nbf, nat.ptr(fd), nat.ptr(od), nat.ptr(pool),
                  nat.stream_ptr())
-        return CompactSupermatrix(pool, fd, od, int((fmt == 0).sum()), int((fmt == 1).sum()),
+        # FP32 maxima rounded up, so a screen never undershoots a tile's bound
+        t32 = torch.from_numpy(np.nextafter(tm.astype(np.float32), np.float32(np.inf))).to(self.dev)
+        return CompactSupermatrix(pool, fd, od, t32, int((fmt == 0).sum()), int((fmt == 1).sum()),
                                   int((fmt == 2).sum()), total, tau_f32, tau_zero)
 
     # -- Fock builds --------------------------------------------------------
@@ -140,11 +153,15 @@ class ERIStore:
             self._work[nrhs] = torch.empty(max(1, wb // 8), dtype=torch.float64, device=self.dev)
             self._x[nrhs] = torch.empty(nrhs * self.npad, dtype=torch.float64, device=self.dev)
             self._y[nrhs] = torch.empty(nrhs * self.npad, dtype=torch.float64, device=self.dev)
+            self._xm[nrhs] = torch.empty(self.T, dtype=torch.float32, device=self.dev)
         return self._work[nrhs], self._x[nrhs], self._y[nrhs]
 
     def contract(self, M: torch.Tensor, D: torch.Tensor, H: torch.Tensor | None = None,
-                 scale: float = 1.0) -> torch.Tensor:
-        """Return H + scale * M D~ for a density or a stack of densities."""
+                 scale: float = 1.0, tau_skip: float = 0.0) -> torch.Tensor:
+        """Return H + scale * M D~ for a density or a stack of densities.
+        For a compacted M, tiles with max|M_tile| max|D~ block| < tau_skip
+        are left out (each moves an element by less than 64 tau_skip); zero
+        tiles are always left out."""
         D = D.contiguous()
         squeeze = D.dim() == 2
         if squeeze:
@@ -154,9 +171,16 @@ class ERIStore:
         s = nat.stream_ptr()
         nat.call("qc_pack_density", nat.ptr(D), self.nbf, nrhs, nat.ptr(x), s)
         if isinstance(M, CompactSupermatrix):
-            nat.call("qc_symv_mixed_tiles", nat.ptr(M.pool), nat.ptr(M.fmt), nat.ptr(M.off),
-                     self.nbf, nrhs, nat.ptr(x), nat.ptr(work), s)
-            nat.call("qc_symv_reduce", self.nbf, nrhs, nat.ptr(y), nat.ptr(work), s)
+            xm = self._xm[nrhs]
+            tau = float(tau_skip)
+            nat.call("qc_block_maxabs", nat.ptr(x), self.nbf, nrhs, nat.ptr(xm), s)
+            nat.call("qc_symv_screened_tiles", nat.ptr(M.pool), nat.ptr(M.fmt), nat.ptr(M.off),
+                     nat.ptr(M.tmax), self.nbf, nrhs, nat.ptr(x), nat.ptr(xm), tau,
+                     nat.ptr(work), s)
+            nat.call("qc_symv_screened_reduce", self.nbf, nrhs, nat.ptr(y), nat.ptr(work),
+                     nat.ptr(M.fmt), nat.ptr(M.tmax), nat.ptr(xm), tau, s)
+        elif tau_skip > 0.0:
+            raise ValueError("tile screening needs a compacted supermatrix (use_compact)")
         else:
             nat.call("qc_symv", nat.ptr(M), self.nbf, nrhs, nat.ptr(x), nat.ptr(y), nat.ptr(work), s)
         F = torch.empty_like(D)
@@ -165,9 +189,57 @@ class ERIStore:
                  nat.ptr(F), s)
         return F[0] if squeeze else F
 
+    # -- incremental builds (mixed path) -------------------------------------
+    def set_incremental(self, tau: float | None = 1e-11, full_every: int = 16):
+        """Build each Fock matrix as G(D) = G(D_prev) + G(D - D_prev), the
+        difference contracted with tile screening at ``tau`` (the standard
+        incremental build of direct SCF: as the SCF converges, D - D_prev
+        shrinks and most tiles drop out).  A full build is made on the first
+        call, after ``full_every`` incremental ones, and whenever
+        ``request_full_build`` asks (the SCF drivers do so before they accept
+        convergence, so a converged result always rests on a full build).
+        Needs compacted supermatrices; tau=None switches it off."""
+        self.inc = None if tau is None else _Incremental(float(tau), int(full_every))
+        return self
+
+    def request_full_build(self):
+        if self.inc is not None:
+            self.inc.force_full = True
+
+    def last_build_incremental(self) -> bool:
+        return self.inc is not None and self.inc.last_incremental
+
+    def _inc_build(self, dens, full_fn, inc_fn):
+        inc = self.inc
+        full = (inc.prev is None or inc.force_full or inc.since_full >= inc.full_every)
+        if full:
+            G = full_fn(dens)
+            inc.since_full = 0
+            inc.n_full += 1
+        else:
+            dG = inc_fn([d - p for d, p in zip(dens, inc.prev)])
+            G = [g + dg for g, dg in zip(inc.G, dG)]
+            inc.since_full += 1
+            inc.n_inc += 1
+        inc.force_full = False
+        inc.last_incremental = not full
+        inc.prev = [d.clone() for d in dens]
+        inc.G = G
+        return G
+
     def fock_rhf(self, H, D):
         """F = H + J(D) - 1/2 K(D) from the P supermatrix."""
+        if self.inc is not None:
+            P = self._compact_or_raise(self.ensure_P())
+            G = self._inc_build([D], lambda d: [self.contract(P, d[0])],
+                                lambda d: [self.contract(P, d[0], tau_skip=self.inc.tau)])
+            return H + G[0]
         return self.contract(self.ensure_P(), D, H)
+
+    def _compact_or_raise(self, M):
+        if not isinstance(M, CompactSupermatrix):
+            raise RuntimeError("incremental Fock builds need compacted supermatrices (use_compact)")
+        return M
 
     def coulomb(self, D):
         if self.J is None:
@@ -179,6 +251,17 @@ class ERIStore:
 
     def fock_uhf(self, H, Da, Db):
         """F_s = H + J(Da + Db) - K(D_s)."""
+        if self.inc is not None:
+            J = self._compact_or_raise(self.J)
+            K = self._compact_or_raise(self.ensure_K())
+
+            def build(d, tau=0.0):
+                Gj = self.contract(J, d[0] + d[1], tau_skip=tau)
+                Gk = self.contract(K, torch.stack(d), tau_skip=tau)
+                return [Gj - Gk[0], Gj - Gk[1]]
+
+            G = self._inc_build([Da, Db], build, lambda d: build(d, self.inc.tau))
+            return H + G[0], H + G[1]
         Jf = self.coulomb_plus(H, Da + Db)
         K = self.contract(self.ensure_K(), torch.stack([Da, Db]))
         return Jf - K[0], Jf - K[1]
@@ -187,3 +270,16 @@ class ERIStore:
         if self.J is None:
             raise RuntimeError("J supermatrix was released")
         return self.contract(self.J, D, H)
+
+
+@dataclass
+class _Incremental:
+    tau: float
+    full_every: int
+    prev: list | None = None
+    G: list | None = None
+    since_full: int = 0
+    force_full: bool = False
+    last_incremental: bool = False
+    n_full: int = 0
+    n_inc: int = 0

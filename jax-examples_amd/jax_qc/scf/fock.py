@@ -51,3 +51,14 @@ def build_fock_uhf(H_core, D_alpha, D_beta, ERI):
     _, Ka = _dense_jk(D_alpha, ERI)
     _, Kb = _dense_jk(D_beta, ERI)
     return H_core + J - Ka, H_core + J - Kb
+
+
+def last_build_was_full(ERI) -> bool:
+    """True unless ERI is an ERIStore in incremental mode whose last Fock
+    build was incremental; in that case the next build is made in full (the
+    SCF drivers call this before accepting convergence, so a converged
+    result always rests on a full build)."""
+    if isinstance(ERI, ERIStore) and ERI.last_build_incremental():
+        ERI.request_full_build()
+        return False
+    return True

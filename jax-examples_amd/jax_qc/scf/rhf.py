@@ -22,7 +22,7 @@ from jax_qc.scf.damping import damp
 from jax_qc.scf.density import density_rhf
 from jax_qc.scf.diis import diis_extrapolate, diis_history_init
 from jax_qc.scf.energy import electronic_energy_rhf
-from jax_qc.scf.fock import build_fock_rhf
+from jax_qc.scf.fock import build_fock_rhf, last_build_was_full
 from jax_qc.scf.guess import core_guess
 from jax_qc.scf.orthogonalize import symmetric_orthogonalization
 from jax_qc.scf import purify
@@ -88,6 +88,8 @@ def run_rhf(mol: Molecule, ints: IntegralSet, config: CalcConfig = CalcConfig(),
             err_max = float(err.abs().max())
             last = state.energy
             if dE < config.scf_conv and err_max < max(config.scf_conv, 1e-6):
+                if not last_build_was_full(ints.ERI):
+                    continue
                 converged = True
                 break
         if state.coefficients is None:

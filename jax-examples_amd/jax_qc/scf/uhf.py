@@ -20,7 +20,7 @@ from jax_qc.scf.damping import damp
 from jax_qc.scf.density import density_uhf
 from jax_qc.scf.diis import diis_extrapolate_uhf, diis_history_init_uhf
 from jax_qc.scf.energy import electronic_energy_uhf
-from jax_qc.scf.fock import build_fock_uhf
+from jax_qc.scf.fock import build_fock_uhf, last_build_was_full
 from jax_qc.scf.guess import core_guess_uhf
 from jax_qc.scf.orthogonalize import symmetric_orthogonalization
 from jax_qc.scf import purify
@@ -106,6 +106,8 @@ def run_uhf(mol: Molecule, ints: IntegralSet, config: CalcConfig = CalcConfig(),
             dE = abs(state.energy - last)
             last = state.energy
             if dE < config.scf_conv and err < max(config.scf_conv, 1e-6):
+                if not last_build_was_full(ints.ERI):
+                    continue
                 converged = True
                 break
         if state.coefficients_alpha is None:

@@ -144,6 +144,19 @@ def eri_tensor(basis, mode=BOYS_ACCURATE) -> np.ndarray:
     return out
 
 
+def eri_tensor_touching(basis, atom: int, mode=BOYS_ACCURATE) -> np.ndarray:
+    """eri_tensor with only the canonical quartets that hold a shell on
+    ``atom`` computed (zeros elsewhere): the elements that move when that
+    atom is displaced."""
+    ob = OracleBasis(basis)
+    n = ob.n_basis
+    sat = np.ascontiguousarray([s.atom_index for s in basis.shells], dtype=np.int32)
+    out = np.zeros((n, n, n, n))
+    lib().orc_eri_tensor_atom(ctypes.c_int(mode), *ob.args(), ctypes.c_int(int(ob.spherical)),
+                              _p(out), _p(sat), ctypes.c_int(int(atom)))
+    return out
+
+
 def contracted_eri_block(basis, i, j, k, l, mode=BOYS_ACCURATE) -> np.ndarray:
     """jax_qc/integrals/obara_saika.py:394-426 (Cartesian block)."""
     ob = OracleBasis(basis)

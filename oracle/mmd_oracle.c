@@ -532,23 +532,38 @@ static int shell_offsets(const orc_basis* bs, int sph, int* off) {
 
 /* jax_qc/integrals/eri.py:69-138 (compute_eri_tensor, _scatter_8fold):
  * canonical quartets i>=j, k>=l, ij>=kl, optional spherical transform,
- * scatter to all 8 permutations of a dense (n,n,n,n) tensor. */
-void orc_eri_tensor(int mode, int nshell, const int* l, const int* nprim, const int* prim_off,
-                    const double* exps, const double* coefs, const double* centers, int sph,
-                    double* eri) {
+ * scatter to all 8 permutations of a dense (n,n,n,n) tensor.  With shell_atom
+ * non-NULL only the canonical quartets that hold a shell on atom `atom` are
+ * computed (the rest of the tensor is zero): those are the only elements
+ * that move when that atom is displaced, so a finite difference of two such
+ * tensors equals the difference of two full rebuilds (the reference's
+ * jax_qc/grad/rhf_grad.py:153-209) with the untouched elements' exact zero
+ * differences left out. */
+static void eri_tensor_impl(int mode, int nshell, const int* l, const int* nprim,
+                            const int* prim_off, const double* exps, const double* coefs,
+                            const double* centers, int sph, double* eri, const int* shell_atom,
+                            int atom) {
   orc_basis bs = {nshell, l, nprim, prim_off, exps, coefs, centers};
   int* off = malloc(sizeof(int) * (nshell + 1));
   int64_t n = shell_offsets(&bs, sph, off);
   memset(eri, 0, sizeof(double) * n * n * n * n);
-  double* blk = malloc(sizeof(double) * 1296);
-  double Cm[4][30];
-  for (int i = 0; i < nshell; ++i)
-    for (int j = 0; j <= i; ++j) {
-      int ij = i * (i + 1) / 2 + j;
+  /* canonical quartets write disjoint sets of elements, so bra pairs are
+   * shared out over threads (the loop order within a pair is unchanged) */
+  const int npair = nshell * (nshell + 1) / 2;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int ij = npair - 1; ij >= 0; --ij) {
+      int i = 0;
+      while ((i + 1) * (i + 2) / 2 <= ij) ++i;
+      int j = ij - i * (i + 1) / 2;
+      double blk[1296];
+      double Cm[4][30];
       for (int k = 0; k < nshell; ++k)
         for (int ll = 0; ll <= k; ++ll) {
           int kl = k * (k + 1) / 2 + ll;
           if (ij < kl) continue;
+          if (shell_atom && shell_atom[i] != atom && shell_atom[j] != atom &&
+              shell_atom[k] != atom && shell_atom[ll] != atom)
+            continue;
           contracted_eri(mode, &bs, i, j, k, ll, blk);
           int d[4] = {ncart(l[i]), ncart(l[j]), ncart(l[k]), ncart(l[ll])};
           if (sph) {
@@ -577,9 +592,22 @@ void orc_eri_tensor(int mode, int nshell, const int* l, const int* nprim, const 
 #undef PUT
                 }
         }
-    }
-  free(blk);
+  }
   free(off);
+}
+
+void orc_eri_tensor(int mode, int nshell, const int* l, const int* nprim, const int* prim_off,
+                    const double* exps, const double* coefs, const double* centers, int sph,
+                    double* eri) {
+  eri_tensor_impl(mode, nshell, l, nprim, prim_off, exps, coefs, centers, sph, eri, NULL, -1);
+}
+
+void orc_eri_tensor_atom(int mode, int nshell, const int* l, const int* nprim,
+                         const int* prim_off, const double* exps, const double* coefs,
+                         const double* centers, int sph, double* eri, const int* shell_atom,
+                         int atom) {
+  eri_tensor_impl(mode, nshell, l, nprim, prim_off, exps, coefs, centers, sph, eri, shell_atom,
+                  atom);
 }
 
 /* Shell-pair Schwarz bound Q_ij = sqrt(max_ab (ab|ab)) over Cartesian

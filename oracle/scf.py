@@ -212,10 +212,15 @@ def _nuc_rep(coords, Z):
     return tot
 
 
-def rhf_gradient(mol, res, basis_fn, displace_fn, step=1e-4, mode=mmd.BOYS_ACCURATE):
+def rhf_gradient(mol, res, basis_fn, displace_fn, step=1e-4, mode=mmd.BOYS_ACCURATE,
+                 touching_only=False):
     """FD-of-integrals Hellmann-Feynman + Pulay gradient at fixed D, W
     (rhf_grad.py:106-211).  ``basis_fn(mol)`` rebuilds the basis,
-    ``displace_fn(mol, coords)`` returns the molecule at new coordinates."""
+    ``displace_fn(mol, coords)`` returns the molecule at new coordinates.
+    touching_only: the two ERI rebuilds of a displaced atom compute only the
+    quartets with a shell on that atom (every other element is computed from
+    identical inputs at both geometries, so its difference is exactly zero);
+    the result is the full-rebuild one up to summation order."""
     coords = np.asarray(mol.coords, dtype=np.float64)
     Z = np.asarray(mol.atomic_numbers, dtype=np.float64)
     D = res["D"]
@@ -228,8 +233,17 @@ def rhf_gradient(mol, res, basis_fn, displace_fn, step=1e-4, mode=mmd.BOYS_ACCUR
             cm[a, k] -= step
             dEn = (_nuc_rep(cp, Z) - _nuc_rep(cm, Z)) / (2 * step)
             mp, mm = displace_fn(mol, cp), displace_fn(mol, cm)
-            ip = mmd.compute_integrals(mp, basis_fn(mp), mode)
-            im = mmd.compute_integrals(mm, basis_fn(mm), mode)
+            if touching_only:
+                bp, bm = basis_fn(mp), basis_fn(mm)
+                ip = dict(S=mmd.one_electron(bp, "S"),
+                          H_core=mmd.one_electron(bp, "T") + mmd.one_electron(bp, "V", mp, mode),
+                          ERI=mmd.eri_tensor_touching(bp, a, mode))
+                im = dict(S=mmd.one_electron(bm, "S"),
+                          H_core=mmd.one_electron(bm, "T") + mmd.one_electron(bm, "V", mm, mode),
+                          ERI=mmd.eri_tensor_touching(bm, a, mode))
+            else:
+                ip = mmd.compute_integrals(mp, basis_fn(mp), mode)
+                im = mmd.compute_integrals(mm, basis_fn(mm), mode)
             dH = (ip["H_core"] - im["H_core"]) / (2 * step)
             dS = (ip["S"] - im["S"]) / (2 * step)
             dERI = (ip["ERI"] - im["ERI"]) / (2 * step)

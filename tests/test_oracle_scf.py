@@ -71,3 +71,34 @@ def test_density_trace_and_energy_weighted_density():
     assert float(np.trace(r["D"] @ ints["S"])) == pytest.approx(mol.n_electrons, abs=1e-10)
     W = scf.energy_weighted_density(r["C"], r["eps"], r["n_occ"])
     assert float(np.trace(W @ ints["S"])) == pytest.approx(2 * r["eps"][: r["n_occ"]].sum(), abs=1e-10)
+
+
+def test_fd_gradient_touching_only_equals_full_rebuilds():
+    """The oracle's FD-of-integrals gradient with atom-restricted ERI
+    rebuilds (used to write the (H2O)_4 fixture) equals the one from full
+    rebuilds (jax_qc/grad/rhf_grad.py:153-209), and a planted error in the
+    restricted builds (one atom's quartets left out) is caught."""
+    import numpy as np
+
+    from jax_qc.core.types import make_molecule
+
+    def disp(m, c):
+        return make_molecule(m.elements, c, m.atomic_numbers, m.charge, m.spin)
+
+    def bf(m):
+        return build_basis_set(m, "6-31g")
+
+    mol = M.h2o_tier2()
+    res = scf.run_rhf(mol, mmd.compute_integrals(mol, bf(mol)), conv=1e-12)
+    g_full = scf.rhf_gradient(mol, res, bf, disp, step=1e-4)
+    g_touch = scf.rhf_gradient(mol, res, bf, disp, step=1e-4, touching_only=True)
+    np.testing.assert_allclose(g_touch, g_full, atol=1e-10)
+    np.testing.assert_allclose(g_full.sum(axis=0), 0.0, atol=1e-7)
+    # planted mistake: restrict to the wrong atom for atom 0's rebuilds
+    orig = mmd.eri_tensor_touching
+    try:
+        mmd.eri_tensor_touching = lambda b, a, mode=mmd.BOYS_ACCURATE: orig(b, (a + 1) % 3, mode)
+        g_bad = scf.rhf_gradient(mol, res, bf, disp, step=1e-4, touching_only=True)
+    finally:
+        mmd.eri_tensor_touching = orig
+    assert np.abs(g_bad - g_full).max() > 1e-3
