@@ -37,14 +37,16 @@ class OptResult(NamedTuple):
 
 def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12, tau32=None,
                         tile_f32=None, grad_tau=1e-13, grad_tau_density=0.0, grad_tau32=None,
-                        prim_eps=None, fock_tau=None, fock_full_every=16, timings=None):
+                        prim_eps=None, fock_tau=None, fock_full_every=0, timings=None,
+                        keep_store=False):
     """Energy and gradient at ``coords``.  Mixed-path options: tau32 (FP32
     ERI sums below that Schwarz bound), tile_f32 (FP32 supermatrix tiles),
     prim_eps (primitive-pair screening of the ERIs and derivative ERIs),
     fock_tau (incremental Fock builds with tile screening), grad_tau_density
     and grad_tau32 (gradient screening / FP32 sums).  ``timings`` (a dict)
     receives the wall time of each stage, the device synchronised between
-    them."""
+    them.  keep_store: the returned result keeps its integral store
+    (``res.ERI``) for further Fock builds."""
     import time
 
     def mark(name, t0):
@@ -76,6 +78,8 @@ def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12,
     if timings is not None and hasattr(ints.ERI, "inc") and ints.ERI.inc is not None:
         timings["fock_full_builds"] = ints.ERI.inc.n_full
         timings["fock_incremental_builds"] = ints.ERI.inc.n_inc
+    if keep_store:
+        return float(res.energy), g.cpu().numpy(), m, res
     # drop the integral store (tens of GB at 480 functions) before the
     # caller's next evaluation
     res = dataclasses.replace(res, ERI=None)

@@ -75,6 +75,55 @@ def _densities(mol, res):
     return D, [D], 0.25, W
 
 
+# Drop the quartets that the kernels' density screen would skip before the
+# launch (see prescreen_launch); the gradient is the same either way.
+PRESCREEN = True
+
+
+def _shell_block_max(M, pt, shell_of_cart):
+    """(nshell, nshell) max |M| over each pair of shells' Cartesian block."""
+    ns = pt.nshell
+    idx = (shell_of_cart[:, None] * ns + shell_of_cart[None, :]).reshape(-1)
+    out = torch.zeros(ns * ns, dtype=M.dtype, device=M.device)
+    out.scatter_reduce_(0, idx, M.abs().reshape(-1), reduce="amax")
+    return out.reshape(ns, ns)
+
+
+def prescreen_launch(L, pt, shells, Mj, Mk, cx, Qd, tau_density):
+    """A copy of class launch L without the quartets (ab|cd) whose bound
+    (Mj[a,b] Mj[c,d] + cx sum_s (Mk_s[a,c] Mk_s[b,d] + Mk_s[a,d] Mk_s[b,c]))
+    * Q_ab Q_cd, with Mj, Mk_s the shell-block maxima of |Dj|, |Dk_s|, is
+    below tau_density / (1 + 1e-6): that bound is at least the max |G_abcd|
+    the kernels compute (in FP64 or FP32), so each quartet left out is one
+    the kernel's own density screen skips, and the gradient is unchanged --
+    only the waves that would have exited after forming G are not launched.
+    The surviving kets of each bra are listed explicitly (ket_list indexed
+    from kstart = the bra's offset)."""
+    from jax_qc.integrals.engine import ClassLaunch
+
+    dev = L.bra.device
+    nq = L.nquartets
+    cnt = (L.offsets[1:] - L.offsets[:-1])
+    bidx = torch.repeat_interleave(torch.arange(L.nbra, device=dev), cnt)
+    local = torch.arange(nq, device=dev, dtype=torch.int64) - L.offsets[bidx]
+    bp = L.bra.long()[bidx]
+    kp = L.ket.long()[L.kstart.long()[bidx] + local]
+    a, b = shells[bp, 0], shells[bp, 1]
+    c, d = shells[kp, 0], shells[kp, 1]
+    bound = Mj[a, b] * Mj[c, d]
+    for M in Mk:
+        bound = bound + cx * (M[a, c] * M[b, d] + M[a, d] * M[b, c])
+    keep = bound * Qd[bp] * Qd[kp] * (1.0 + 1e-6) >= tau_density
+    del a, b, c, d, bound
+    kept = int(keep.sum())
+    ncnt = torch.bincount(bidx[keep], minlength=L.nbra)
+    offs = torch.zeros(L.nbra + 1, dtype=torch.int64, device=dev)
+    offs[1:] = torch.cumsum(ncnt, 0)
+    ket = kp[keep].to(torch.int32)
+    return ClassLaunch(L.cab, L.ccd, L.fp32, L.nbra, kept, L.bra, ket, offs,
+                       offs[:-1].to(torch.int32).contiguous())
+
+
 def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_table=None,
                         parts=("1e", "2e"), tau_density: float = 0.0,
                         tau32: float | None = None) -> torch.Tensor:
@@ -113,7 +162,17 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
     if "2e" in parts:
         plan = eng.plan_quartets(pt, tau, tau32)
         Qd = torch.from_numpy(pt.Q).to(dev) if tau_density > 0.0 else None
+        pre = None
+        if Qd is not None and PRESCREEN:
+            soc = torch.from_numpy(np.repeat(np.arange(pt.nshell),
+                                             np.diff(pt.cart_off)).astype(np.int64)).to(dev)
+            pre = (torch.from_numpy(pt.sp_shells.astype(np.int64)).to(dev),
+                   _shell_block_max(Djc, pt, soc), [_shell_block_max(x, pt, soc) for x in Dkc])
         for L in plan.launches:
+            if pre is not None:
+                L = prescreen_launch(L, pt, pre[0], pre[1], pre[2], cx, Qd, tau_density)
+                if L.nquartets == 0:
+                    continue
             a = nat.QcGrad2eArgs()
             a.pairs64 = pt.pairs64.data_ptr()
             a.sp_prim_off = pt.sp_prim_off.data_ptr()

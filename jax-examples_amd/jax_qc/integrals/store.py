@@ -190,28 +190,32 @@ class ERIStore:
         return F[0] if squeeze else F
 
     # -- incremental builds (mixed path) -------------------------------------
-    def set_incremental(self, tau: float | None = 1e-11, full_every: int = 16):
+    def set_incremental(self, tau: float | None = 1e-12, full_every: int = 0):
         """Build each Fock matrix as G(D) = G(D_prev) + G(D - D_prev), the
         difference contracted with tile screening at ``tau`` (the standard
         incremental build of direct SCF: as the SCF converges, D - D_prev
         shrinks and most tiles drop out).  A full build is made on the first
-        call, after ``full_every`` incremental ones, and whenever
-        ``request_full_build`` asks (the SCF drivers do so before they accept
-        convergence, so a converged result always rests on a full build).
-        Needs compacted supermatrices; tau=None switches it off."""
+        call, after ``full_every`` incremental ones (0: never on a count),
+        and from the first ``request_full_build`` on every build is full (the
+        SCF drivers ask before they accept convergence, so a converged result
+        always rests on full builds, and the screening errors of the
+        incremental phase cannot hold the convergence test off).  Needs
+        compacted supermatrices; tau=None switches it off."""
         self.inc = None if tau is None else _Incremental(float(tau), int(full_every))
         return self
 
     def request_full_build(self):
         if self.inc is not None:
             self.inc.force_full = True
+            self.inc.final = True
 
     def last_build_incremental(self) -> bool:
         return self.inc is not None and self.inc.last_incremental
 
     def _inc_build(self, dens, full_fn, inc_fn):
         inc = self.inc
-        full = (inc.prev is None or inc.force_full or inc.since_full >= inc.full_every)
+        full = (inc.prev is None or inc.force_full or inc.final
+                or (inc.full_every > 0 and inc.since_full >= inc.full_every))
         if full:
             G = full_fn(dens)
             inc.since_full = 0
@@ -280,6 +284,7 @@ class _Incremental:
     G: list | None = None
     since_full: int = 0
     force_full: bool = False
+    final: bool = False
     last_incremental: bool = False
     n_full: int = 0
     n_inc: int = 0

@@ -157,6 +157,16 @@ def eri_tensor_touching(basis, atom: int, mode=BOYS_ACCURATE) -> np.ndarray:
     return out
 
 
+def eri_quartets_checksum(basis, quartets, mode=BOYS_ACCURATE) -> float:
+    """Contracted Cartesian blocks of the (nq, 4) shell quartets (OpenMP
+    over quartets); returns the sum of all elements (for timing runs)."""
+    ob = OracleBasis(basis)
+    q = np.ascontiguousarray(quartets, dtype=np.int32)
+    f = lib().orc_eri_quartets
+    f.restype = ctypes.c_double
+    return float(f(ctypes.c_int(mode), *ob.args(), ctypes.c_int64(len(q)), _p(q)))
+
+
 def contracted_eri_block(basis, i, j, k, l, mode=BOYS_ACCURATE) -> np.ndarray:
     """jax_qc/integrals/obara_saika.py:394-426 (Cartesian block)."""
     ob = OracleBasis(basis)

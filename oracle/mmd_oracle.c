@@ -482,6 +482,27 @@ void orc_contracted_eri(int mode, int nshell, const int* l, const int* nprim, co
   contracted_eri(mode, &bs, i, j, k, ll, out);
 }
 
+/* contracted_eri over a list of nq shell quartets (quartets[4q..4q+3]),
+ * threads sharing the list; returns the sum of every Cartesian element (a
+ * checksum, so the work cannot be elided).  Used to time the restatement of
+ * the reference's quartet loop (jax_qc/integrals/eri.py:89-104) on a sample
+ * of a workload's quartets. */
+double orc_eri_quartets(int mode, int nshell, const int* l, const int* nprim, const int* prim_off,
+                        const double* exps, const double* coefs, const double* centers,
+                        int64_t nq, const int* quartets) {
+  orc_basis bs = {nshell, l, nprim, prim_off, exps, coefs, centers};
+  double total = 0.0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : total)
+  for (int64_t q = 0; q < nq; ++q) {
+    double blk[1296];
+    const int* s = quartets + 4 * q;
+    contracted_eri(mode, &bs, s[0], s[1], s[2], s[3], blk);
+    const int n = ncart(l[s[0]]) * ncart(l[s[1]]) * ncart(l[s[2]]) * ncart(l[s[3]]);
+    for (int x = 0; x < n; ++x) total += blk[x];
+  }
+  return total;
+}
+
 /* jax_qc/integrals/spherical.py:34-67: rows follow libcint order (d_xy,
  * d_yz, d_z2, d_xz, d_x2-y2); l = 1 is the identity. */
 static void cart_to_sph(int l, double* Cm) {

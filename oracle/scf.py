@@ -213,20 +213,21 @@ def _nuc_rep(coords, Z):
 
 
 def rhf_gradient(mol, res, basis_fn, displace_fn, step=1e-4, mode=mmd.BOYS_ACCURATE,
-                 touching_only=False):
+                 touching_only=False, atoms=None):
     """FD-of-integrals Hellmann-Feynman + Pulay gradient at fixed D, W
     (rhf_grad.py:106-211).  ``basis_fn(mol)`` rebuilds the basis,
     ``displace_fn(mol, coords)`` returns the molecule at new coordinates.
     touching_only: the two ERI rebuilds of a displaced atom compute only the
     quartets with a shell on that atom (every other element is computed from
     identical inputs at both geometries, so its difference is exactly zero);
-    the result is the full-rebuild one up to summation order."""
+    the result is the full-rebuild one up to summation order.  atoms: only
+    these atoms' rows are computed (the rest stay zero)."""
     coords = np.asarray(mol.coords, dtype=np.float64)
     Z = np.asarray(mol.atomic_numbers, dtype=np.float64)
     D = res["D"]
     W = energy_weighted_density(res["C"], res["eps"], mol.n_electrons // 2)
     g = np.zeros_like(coords)
-    for a in range(len(Z)):
+    for a in (range(len(Z)) if atoms is None else atoms):
         for k in range(3):
             cp, cm = coords.copy(), coords.copy()
             cp[a, k] += step

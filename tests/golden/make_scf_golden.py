@@ -20,8 +20,9 @@ test:
 Each file also holds the input coordinates, so a test can check that the
 generators still produce the same molecules.
 
-Run: python tests/golden/make_scf_golden.py [water4|water2|batch6 ...]
-(the water4 gradient takes about half an hour on 8 cores).
+Run: python tests/golden/make_scf_golden.py [water4|water2|batch6 ...] [--cache=DIR]
+(the water4 gradient takes about an hour on 8 cores; --cache keeps per-atom
+rows so an interrupted run resumes).
 """
 
 import json
@@ -63,13 +64,27 @@ def _write(name, obj):
     print("wrote", path, flush=True)
 
 
-def water4():
+def water4(cache=None):
+    """cache: a directory for per-atom rows, so an interrupted run resumes."""
     mol = M.water_cluster((2, 2, 1), seed=1)
     bs = _bf(mol)
     t = time.time()
     res = scf.run_rhf(mol, mmd.compute_integrals(mol, bs), conv=1e-12)
     assert res["converged"]
-    g = scf.rhf_gradient(mol, res, _bf, _displace, step=1e-4, touching_only=True)
+    g = np.zeros((len(mol.atomic_numbers), 3))
+    for a in range(len(mol.atomic_numbers)):
+        path = os.path.join(cache, f"water4_atom{a}.json") if cache else None
+        if path and os.path.exists(path):
+            with open(path) as fh:
+                g[a] = json.load(fh)
+            continue
+        g[a] = scf.rhf_gradient(mol, res, _bf, _displace, step=1e-4, touching_only=True,
+                                atoms=[a])[a]
+        print(f"atom {a}: {g[a].tolist()}  ({time.time() - t:.0f} s)", flush=True)
+        if path:
+            os.makedirs(cache, exist_ok=True)
+            with open(path, "w") as fh:
+                json.dump(g[a].tolist(), fh)
     _write("water4_rhf_grad.json", {
         "molecule": _mol_record(mol), "basis": "even-tempered", "n_basis": bs.n_basis,
         "scf_conv": 1e-12, "energy": res["energy"], "n_iter": res["n_iter"], "fd_step": 1e-4,
@@ -99,6 +114,11 @@ def batch6():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["water2", "batch6", "water4"]
+    args = [a for a in sys.argv[1:] if not a.startswith("--cache=")]
+    cache = next((a.split("=", 1)[1] for a in sys.argv[1:] if a.startswith("--cache=")), None)
+    which = args or ["water2", "batch6", "water4"]
     for w in which:
-        {"water4": water4, "water2": water2, "batch6": batch6}[w]()
+        if w == "water4":
+            water4(cache)
+        else:
+            {"water2": water2, "batch6": batch6}[w]()

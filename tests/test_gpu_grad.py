@@ -157,3 +157,30 @@ def test_thread_kernels_lds_staged_adds_equal_global_adds():
         an.GRAD_VARIANT_OVERRIDE = None
         an.GRAD_STAGE_IN_LDS = True
     np.testing.assert_allclose(g[True], g[False], atol=1e-12)
+
+
+@pytest.mark.parametrize("method", ["rhf", "uhf"])
+def test_density_prescreen_leaves_gradient_unchanged(method):
+    """The launch-side density prescreen (shell-block density maxima) drops
+    only quartets the kernels' own screen skips: the gradient with it equals
+    the gradient without it, in FP64 and with the FP32 split, RHF and UHF
+    ((H2O)_2 even-tempered with d shells)."""
+    import jax_qc
+    from jax_qc.grad import analytic as an
+
+    spin = 2 if method == "uhf" else 0
+    mol = M.water_cluster((2, 1, 1), seed=1, spin=spin)
+    bs = build_basis_set(mol, "even-tempered")
+    ints = jax_qc.compute_integrals(mol, bs, eri="store", tau=1e-14)
+    res = jax_qc.run_scf(mol, ints, CalcConfig(method=method, scf_conv=1e-12, max_scf_iter=200))
+    pt = ints.ERI.pt
+    for tau32 in (None, 1e-2):
+        g = {}
+        try:
+            for pre in (True, False):
+                an.PRESCREEN = pre
+                g[pre] = an.scf_gradient(mol, bs, res, 1e-12, pair_table=pt, tau_density=1e-9,
+                                         tau32=tau32).cpu().numpy()
+        finally:
+            an.PRESCREEN = True
+        np.testing.assert_allclose(g[True], g[False], atol=1e-13, rtol=0)
