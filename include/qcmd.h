@@ -88,9 +88,14 @@ int qc_one_electron(int which, int cab, const double* pairs64, const int* sp_pri
  * output are tile-major packed supermatrices over nbf functions. */
 int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* stream);
 
-/* Launch shape of the Fock-build tile pass: tiles per work item (one wave
- * per item) and whether tile loads are non-temporal.  Process-wide. */
+/* Launch shape of the Fock-build tile pass: tiles per work item (1..64) and
+ * whether tile loads are non-temporal.  Process-wide. */
 int qc_symv_config(int seg, int nontemporal);
+
+/* Waves per work item of the tile pass for nrhs right-hand sides (1 or 2;
+ * with 2 each wave takes half of every tile's rows, which halves the
+ * registers per lane).  Process-wide; defaults 1 for nrhs = 1, 2 above. */
+int qc_symv_split(int nrhs, int waves);
 
 /* Workspace (bytes) for qc_symv over nbf functions and nrhs vectors. */
 int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs);

@@ -17,8 +17,17 @@
 namespace qcmd {
 
 // Tiles per SYMV work item and the load flavour; set by qc_symv_config.
-static int g_symv_seg = 4;
+// 16 tiles per item measured best on the (H2O)_16 pools (K pass with two
+// right-hand sides: 6.14 ms at 4, 5.29 at 8, 5.04 at 16, 4.96 at 32; the
+// one-right-hand-side J pass 5.38 / 5.31 / 5.31 / 5.35 ms).
+static int g_symv_seg = 16;
 static int g_symv_nt = 1;
+// Waves per work item by right-hand-side count (qc_symv_split): with one
+// wave a lane holds 16 rows of x and row sums per right-hand side, which at
+// two right-hand sides costs the second wave per SIMD (334 VGPRs); two waves
+// per item halve that (198 VGPRs; K pass 5.40 -> 5.04 ms at 16 tiles per
+// item).  One right-hand side measured the same either way.
+static int g_symv_ws[5] = {1, 1, 2, 2, 2};
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 __host__ __device__ inline int64_t sm_tiles_per_dim(int64_t nbf) {
@@ -82,6 +91,15 @@ __host__ __device__ inline int64_t item_off(int64_t R, int seg) {
   return R + seg * m * (m - 1) / 2 + m * (R - m * seg);
 }
 
+// Bytes of row and column partials one SYMV uses at segment length seg:
+// item_off(T, seg) row-partial blocks and T(T+1)/2 column-partial blocks of
+// 64 doubles per right-hand side.
+static int64_t symv_work_bytes(int64_t nbf, int nrhs, int seg) {
+  const int64_t T = sm_tiles_per_dim(nbf);
+  const int64_t items = item_off(T, seg), tiles = T * (T + 1) / 2;
+  return (items + tiles) * nrhs * SM_B * (int64_t)sizeof(double);
+}
+
 // Tile formats of a compacted (mixed-precision) supermatrix.
 enum TileFmt : uint8_t { TILE_ZERO = 0, TILE_F32 = 1, TILE_F64 = 2 };
 typedef float fvec2 __attribute__((ext_vector_type(2)));
@@ -112,19 +130,48 @@ __device__ __forceinline__ int lane_col(int cg, int j) {
   return j < 2 ? 2 * cg + j : 32 + 2 * cg + (j - 2);
 }
 
-// Column partials of one off-diagonal tile: sum over the lane's 16 rows,
-// then over the four row offsets (xor 16, xor 32); lanes with ro == 0 store.
-template <int NRHS, typename T>
+// Column partials of one off-diagonal tile: sum over the lane's rows, then
+// over the four row offsets (xor 16, xor 32); lanes with ro == 0 store.  With
+// two waves per work item (WS = 2, each wave half of the tile's rows) wave 1
+// hands its sums to wave 0 through LDS, double-buffered by the tile's parity
+// so that one barrier per tile suffices: wave 1 writes buffer p for tile t+2
+// only after both waves passed tile t+1's barrier, by which time wave 0 has
+// read tile t's values from it.
+template <int NRHS, int WS>
+struct ColShare {
+  double v[2][NRHS][SM_B];
+};
+template <int NRHS>
+struct ColShare<NRHS, 1> {};
+
+template <int NRHS, int WS, typename T>
 __device__ __forceinline__ void store_cols(double (&col)[NRHS][4], int64_t tile, int cg, int ro,
+                                           int wave, int par, ColShare<NRHS, WS>& sh,
                                            double* __restrict__ colbuf) {
 #pragma unroll
-  for (int h = 0; h < NRHS; ++h) {
+  for (int h = 0; h < NRHS; ++h)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       col[h][j] += __shfl_xor(col[h][j], 16);
       col[h][j] += __shfl_xor(col[h][j], 32);
     }
-    if (ro == 0) {
+  if constexpr (WS == 2) {
+    if (wave == 1 && ro == 0) {
+#pragma unroll
+      for (int h = 0; h < NRHS; ++h)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sh.v[par][h][lane_col<T>(cg, j)] = col[h][j];
+    }
+    __syncthreads();
+    if (wave == 1) return;
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) col[h][j] += sh.v[par][h][lane_col<T>(cg, j)];
+  }
+  if (ro == 0) {
+#pragma unroll
+    for (int h = 0; h < NRHS; ++h) {
       double* dst = colbuf + (tile * NRHS + h) * SM_B;
 #pragma unroll
       for (int j = 0; j < 4; ++j) dst[lane_col<T>(cg, j)] = col[h][j];
@@ -132,12 +179,17 @@ __device__ __forceinline__ void store_cols(double (&col)[NRHS][4], int64_t tile,
   }
 }
 
-template <int NRHS, bool NT, typename T>
+// One tile: lane (ro, cg) of wave `wave` covers rows r0 + 4k + ro (k < ROWS,
+// r0 = 64 wave / WS) and the four columns lane_col(cg, j).
+template <int NRHS, int WS, bool NT, typename T>
 __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile, bool diag,
-                                          const double (&xr)[NRHS][SYMV_ROWS],
-                                          double (&rowacc)[NRHS][SYMV_ROWS],
+                                          const double (&xr)[NRHS][SYMV_ROWS / WS],
+                                          double (&rowacc)[NRHS][SYMV_ROWS / WS],
                                           const double* __restrict__ xcb, int64_t npad, int cg,
-                                          int ro, double* __restrict__ colbuf) {
+                                          int ro, int wave, int par, ColShare<NRHS, WS>& sh,
+                                          double* __restrict__ colbuf) {
+  constexpr int ROWS = SYMV_ROWS / WS;
+  const int r0 = (SM_B / WS) * wave;
   double xc[NRHS][4], col[NRHS][4];
 #pragma unroll
   for (int h = 0; h < NRHS; ++h)
@@ -147,8 +199,8 @@ __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile
       col[h][j] = 0.0;
     }
 #pragma unroll
-  for (int k = 0; k < SYMV_ROWS; ++k) {
-    const T* row = tp + (4 * k + ro) * SM_B;
+  for (int k = 0; k < ROWS; ++k) {
+    const T* row = tp + (r0 + 4 * k + ro) * SM_B;
     double m[4];
     if (sizeof(T) == 8) {
       const dvec2 a = ld<NT>(reinterpret_cast<const dvec2*>(row) + cg);
@@ -171,7 +223,7 @@ __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile
       for (int j = 0; j < 4; ++j) col[h][j] += m[j] * xr[h][k];
     }
   }
-  if (!diag) store_cols<NRHS, T>(col, tile, cg, ro, colbuf);
+  if (!diag) store_cols<NRHS, WS, T>(col, tile, cg, ro, wave, par, sh, colbuf);
 }
 
 // Optional screen of a compacted supermatrix's tiles: a tile (R, C) is
@@ -202,7 +254,7 @@ __device__ __forceinline__ bool tile_live(const uint8_t* __restrict__ fmt, const
 // halving over the 16 lanes sharing ro.  MIXED: tiles are read through a
 // format/offset directory (zero tiles skipped, FP32 or FP64, FP64
 // accumulation either way).
-template <int NRHS, bool MIXED, bool NT>
+template <int NRHS, bool MIXED, bool NT, int WS>
 __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T,
                                           const uint8_t* __restrict__ fmt,
                                           const int64_t* __restrict__ off,
@@ -220,17 +272,35 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
   }
   const int64_t R = lo, s = item - item_off(R, seg);
   const int64_t C0 = s * seg, C1 = min(C0 + seg, R + 1);
-  const int lane = threadIdx.x, cg = lane & 15, ro = lane >> 4;
+  constexpr int ROWS = SYMV_ROWS / WS;
+  const int lane = threadIdx.x & 63, cg = lane & 15, ro = lane >> 4;
+  const int wave = WS == 1 ? 0 : (int)(threadIdx.x >> 6);
+  const int r0 = (SM_B / WS) * wave;
+  __shared__ ColShare<NRHS, WS> sh;
+  int par = 0;
 
-  double xr[NRHS][SYMV_ROWS], rowacc[NRHS][SYMV_ROWS];
+  double xr[NRHS][ROWS], rowacc[NRHS][ROWS];
 #pragma unroll
   for (int h = 0; h < NRHS; ++h)
 #pragma unroll
-    for (int k = 0; k < SYMV_ROWS; ++k) {
-      xr[h][k] = x[h * npad + (row0 + R) * SM_B + 4 * k + ro];
+    for (int k = 0; k < ROWS; ++k) {
+      xr[h][k] = x[h * npad + (row0 + R) * SM_B + r0 + 4 * k + ro];
       rowacc[h][k] = 0.0;
     }
   const float xmR = (MIXED && scr.tmax) ? scr.xmax[row0 + R] : 0.0f;
+  // The directory entries (format, pool offset, screen) of the segment's
+  // tiles are loaded at once, lane c holding tile C0 + c, and read back with
+  // uniform lane reads: the tile loop then has no dependent directory load in
+  // front of each tile's data loads (seg <= 64, checked on the host).
+  int dir_f = TILE_ZERO;
+  int64_t dir_o = 0;
+  if (MIXED && lane < C1 - C0) {
+    const int64_t t = tile0 + R * (R + 1) / 2 + C0 + lane;
+    dir_f = fmt[t];
+    if (scr.tmax && dir_f != TILE_ZERO && !tile_live(fmt, scr, t, xmR, scr.xmax[row0 + C0 + lane]))
+      dir_f = TILE_ZERO + 8;  // screened out: no data, no column partials
+    dir_o = off[t];
+  }
 
   for (int64_t C = C0; C < C1; ++C) {
     const int64_t tile = tile0 + R * (R + 1) / 2 + C;
@@ -238,18 +308,22 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
     const double* xcb = x + (row0 + C) * SM_B;
     if (!MIXED) {
       const double* tp = reinterpret_cast<const double*>(Mv) + (tile << (2 * SM_LOGB));
-      symv_tile<NRHS, NT, double>(tp, tile, diag, xr, rowacc, xcb, npad, cg, ro, colbuf);
+      symv_tile<NRHS, WS, NT, double>(tp, tile, diag, xr, rowacc, xcb, npad, cg, ro, wave, par,
+                                      sh, colbuf);
     } else {
-      const uint8_t f = fmt[tile];
-      if (scr.tmax && !tile_live(fmt, scr, tile, xmR, scr.xmax[row0 + C])) continue;
-      const char* base = reinterpret_cast<const char*>(Mv) + off[tile];
+      const int c = (int)(C - C0);
+      const int f = __builtin_amdgcn_readlane(dir_f, c);
+      if (f == TILE_ZERO + 8 || (scr.tmax && f == TILE_ZERO)) continue;
+      const uint32_t olo = __builtin_amdgcn_readlane((uint32_t)dir_o, c);
+      const uint32_t ohi = __builtin_amdgcn_readlane((uint32_t)(dir_o >> 32), c);
+      const char* base = reinterpret_cast<const char*>(Mv) + (int64_t)(((uint64_t)ohi << 32) | olo);
       if (f == TILE_F64) {
-        symv_tile<NRHS, NT, double>(reinterpret_cast<const double*>(base), tile, diag, xr, rowacc,
-                                    xcb, npad, cg, ro, colbuf);
+        symv_tile<NRHS, WS, NT, double>(reinterpret_cast<const double*>(base), tile, diag, xr,
+                                        rowacc, xcb, npad, cg, ro, wave, par, sh, colbuf);
       } else if (f == TILE_F32) {
-        symv_tile<NRHS, NT, float>(reinterpret_cast<const float*>(base), tile, diag, xr, rowacc,
-                                   xcb, npad, cg, ro, colbuf);
-      } else if (!diag && ro == 0) {
+        symv_tile<NRHS, WS, NT, float>(reinterpret_cast<const float*>(base), tile, diag, xr,
+                                       rowacc, xcb, npad, cg, ro, wave, par, sh, colbuf);
+      } else if (!diag && ro == 0 && wave == 0) {
 #pragma unroll
         for (int h = 0; h < NRHS; ++h) {
           dvec2 z;
@@ -260,13 +334,18 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
         }
       }
     }
+    par ^= 1;
   }
-  // recursive halving over the 16 lanes sharing ro: lane cg ends with row 4cg+ro
+  // recursive halving over the 16 lanes sharing ro (log2 ROWS steps, then
+  // plain sums over the remaining lane bits): lane cg ends with the total of
+  // row r0 + 4 (cg / (16 / ROWS)) + ro
+  constexpr int LOGR = ROWS == 16 ? 4 : 3;
+  static_assert(ROWS == 16 || ROWS == 8, "rows per lane");
 #pragma unroll
   for (int h = 0; h < NRHS; ++h) {
 #pragma unroll
-    for (int step = 0; step < 4; ++step) {
-      const int m = 8 >> step, half = 8 >> step;
+    for (int step = 0; step < LOGR; ++step) {
+      const int m = 8 >> step, half = ROWS >> (step + 1);
       const bool up = (cg & m) != 0;
 #pragma unroll
       for (int i = 0; i < half; ++i) {
@@ -275,20 +354,21 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
         rowacc[h][i] = keep + __shfl_xor(send, m);
       }
     }
-    rowbuf[(gitem * NRHS + h) * SM_B + 4 * cg + ro] = rowacc[h][0];
+#pragma unroll
+    for (int step = LOGR; step < 4; ++step) rowacc[h][0] += __shfl_xor(rowacc[h][0], 8 >> step);
+    if ((cg & ((16 / ROWS) - 1)) == 0)
+      rowbuf[(gitem * NRHS + h) * SM_B + r0 + 4 * (cg / (16 / ROWS)) + ro] = rowacc[h][0];
   }
 }
 
-template <int NRHS, bool MIXED, bool NT>
-__global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_tiles_kernel(const void* __restrict__ Mv, int64_t T,
-                                                        const uint8_t* __restrict__ fmt,
-                                                        const int64_t* __restrict__ off,
-                                                        const double* __restrict__ x, int64_t npad,
-                                                        double* __restrict__ rowbuf,
-                                                        double* __restrict__ colbuf, int seg,
-                                                        TileScreen scr) {
+template <int NRHS, bool MIXED, bool NT, int WS>
+__global__ __launch_bounds__(64 * WS, SYMV_MIN_WAVES) void symv_tiles_kernel(
+    const void* __restrict__ Mv, int64_t T, const uint8_t* __restrict__ fmt,
+    const int64_t* __restrict__ off, const double* __restrict__ x, int64_t npad,
+    double* __restrict__ rowbuf, double* __restrict__ colbuf, int seg, TileScreen scr) {
   const int64_t item = blockIdx.x;
-  symv_item<NRHS, MIXED, NT>(Mv, T, fmt, off, x, npad, rowbuf, colbuf, seg, item, item, 0, 0, scr);
+  symv_item<NRHS, MIXED, NT, WS>(Mv, T, fmt, off, x, npad, rowbuf, colbuf, seg, item, item, 0, 0,
+                                 scr);
 }
 
 // Last index i in [0, n) with pre[i] <= v (pre ascending, pre[0] = 0).
@@ -320,8 +400,8 @@ __global__ __launch_bounds__(64, SYMV_MIN_WAVES) void symv_batch_kernel(
   const TileScreen none = {nullptr, nullptr, 0.0f};
   for (int64_t g = flat_block(); g < total; g += stride) {
     const int m = find_seg(item0, nmol, g);
-    symv_item<NRHS, MIXED, NT>(Mv, molT[m], fmt, off, x, npad, rowbuf, colbuf, seg, g - item0[m],
-                               g, tile0[m], row0[m], none);
+    symv_item<NRHS, MIXED, NT, 1>(Mv, molT[m], fmt, off, x, npad, rowbuf, colbuf, seg,
+                                  g - item0[m], g, tile0[m], row0[m], none);
   }
 }
 
@@ -525,17 +605,23 @@ int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* 
 }
 
 int qc_symv_config(int seg, int nontemporal) {
-  if (seg < 1 || seg > 4096) return set_error("qc_symv_config: seg %d", seg);
+  // a work item's directory is held one tile per lane
+  if (seg < 1 || seg > 64) return set_error("qc_symv_config: seg %d outside [1, 64]", seg);
   g_symv_seg = seg;
   g_symv_nt = nontemporal ? 1 : 0;
   return 0;
 }
 
+int qc_symv_split(int nrhs, int waves) {
+  if (nrhs < 1 || nrhs > 4) return set_error("qc_symv_split: nrhs %d outside [1, 4]", nrhs);
+  if (waves != 1 && waves != 2) return set_error("qc_symv_split: waves %d (1 or 2)", waves);
+  g_symv_ws[nrhs] = waves;
+  return 0;
+}
+
 int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs) {
   // sized for the smallest supported segment so any later qc_symv_config fits
-  const int64_t T = sm_tiles_per_dim(nbf);
-  const int64_t items = item_off(T, 1), tiles = T * (T + 1) / 2;
-  return (items + tiles) * nrhs * SM_B * (int64_t)sizeof(double);
+  return symv_work_bytes(nbf, nrhs, 1);
 }
 
 static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const double* x,
@@ -553,9 +639,15 @@ static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const d
   hipStream_t s = (hipStream_t)stream;
   if (phase & 1) {
     switch (nrhs) {
-#define QC_L(N, MX, NT)                                                                     \
-  hipLaunchKernelGGL((symv_tiles_kernel<N, MX, NT>), dim3((unsigned)items), dim3(64), 0, s, \
-                     (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf, seg, scr)
+#define QC_L(N, MX, NT)                                                                        \
+  do {                                                                                         \
+    if (g_symv_ws[N] == 2)                                                                     \
+      hipLaunchKernelGGL((symv_tiles_kernel<N, MX, NT, 2>), dim3((unsigned)items), dim3(128), 0, \
+                         s, (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf, seg, scr);    \
+    else                                                                                       \
+      hipLaunchKernelGGL((symv_tiles_kernel<N, MX, NT, 1>), dim3((unsigned)items), dim3(64), 0,  \
+                         s, (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf, seg, scr);    \
+  } while (0)
 #define QC_S(N)                     \
   case N:                           \
     if (fmt) {                      \
@@ -666,7 +758,7 @@ int qc_batch_symv_tiles(const void* M, const uint8_t* fmt, const int64_t* off, i
                         int nrhs, int seg, const double* x, void* work, void* stream) {
   (void)ntiles;
   if (nrhs < 1 || nrhs > 4) return set_error("qc_batch_symv_tiles: nrhs %d outside [1, 4]", nrhs);
-  if (seg < 1 || seg > 4096) return set_error("qc_batch_symv_tiles: seg %d", seg);
+  if (seg < 1 || seg > 64) return set_error("qc_batch_symv_tiles: seg %d outside [1, 64]", seg);
   if (!M || !molT || !tile0 || !item0 || !row0 || !x || !work || (fmt && !off))
     return set_error("qc_batch_symv_tiles: null pointer");
   if (nmol <= 0 || nitems <= 0) return 0;

@@ -86,3 +86,31 @@ def test_autotune_picks_a_valid_variant(dimer):
     finally:
         eng.ERI_VARIANT.clear()
         eng.ERI_VARIANT.update(saved)
+
+
+@pytest.mark.parametrize("nrhs", [1, 2, 3])
+def test_symv_two_wave_items_match_one_wave(dimer, nrhs):
+    """The tile pass with two waves per work item (each half of a tile's
+    rows, column sums combined through LDS) gives the one-wave result to
+    rounding, on the plain and the compacted (screened) supermatrix."""
+    from jax_qc import _native as nat
+    from jax_qc.integrals.store import ERIStore
+
+    mol, bs = dimer
+    st = ERIStore(bs, tau=1e-12)
+    P = st.ensure_P()
+    C = st.compact(P, tau_f32=1e-6)
+    n = bs.n_basis
+    g = torch.Generator(device="cpu").manual_seed(nrhs)
+    A = torch.randn(nrhs, n, n, generator=g, dtype=torch.float64)
+    D = (A + A.transpose(1, 2)).cuda()
+    out = {}
+    try:
+        for ws in (1, 2):
+            nat.call("qc_symv_split", nrhs, ws)
+            out[ws] = (st.contract(P, D), st.contract(C, D), st.contract(C, D, tau_skip=1e-9))
+    finally:
+        nat.call("qc_symv_split", nrhs, 1 if nrhs == 1 else 2)
+    for a, b in zip(out[1], out[2]):
+        scale = float(a.abs().max())
+        assert float((a - b).abs().max()) <= 1e-13 * scale
