@@ -105,6 +105,18 @@ int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs);
 int qc_symv(const double* M, int64_t nbf, int nrhs, const double* x, double* y, void* work,
             void* stream);
 
+/* qc_symv without a caller workspace and without allocating: the row and
+ * column partials live in a 64 MiB device array of the library (one per GPU,
+ * its address taken at qc_init, held until the process ends).  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops).  Fails with a clear
+ * error when the workspace would exceed the array (qc_symv_workspace_bytes
+ * at the configured segment length; e.g. nbf <= 250 at nrhs = 1 and the
+ * default segment) or when qc_init was not called on the current device.
+ * Calls that overlap on one GPU share the array and race. */
+int qc_symv_noalloc(const double* M, int64_t nbf, int nrhs, const double* x, double* y,
+                    void* stream);
+
 /* The two phases of qc_symv, callable separately (e.g. to time the
  * HBM-streaming tile pass alone): partial sums into `work`, then y. */
 int qc_symv_tiles(const double* M, int64_t nbf, int nrhs, const double* x, void* work,

@@ -134,6 +134,7 @@ int qc_init(int device) {
   std::lock_guard<std::mutex> lk(g_mu);
   if ((int)g_dev.size() < ndev) g_dev.resize(ndev);
   DeviceState& st = g_dev[device];
+  if (symv_scratch_init(device)) return 1;
   if (st.boys64) return 0;
   std::vector<double> t64;
   std::vector<float> t32;

@@ -10,6 +10,8 @@
 //   RHF: F = H + sum_{k>=l} P[(ij),(kl)] D~_kl,  P = (ij|kl) - 1/4[(ik|jl)+(il|jk)]
 // (Raffenetti's supermatrix), so each Fock build is one symmetric
 // matrix-vector product that streams every unique integral once from HBM.
+#include <mutex>
+
 #include "qcmd_device.h"
 #include "qcmd_internal.h"
 #include "qcmd_types.h"
@@ -28,6 +30,23 @@ static int g_symv_nt = 1;
 // per item halve that (198 VGPRs; K pass 5.40 -> 5.04 ms at 16 tiles per
 // item).  One right-hand side measured the same either way.
 static int g_symv_ws[5] = {1, 1, 2, 2, 2};
+
+// Workspace of qc_symv_noalloc: the SYMV's row and column partials, for
+// callers that cannot allocate (graph capture, serving loops).  Each GPU
+// holds its own copy for the life of the process; its address is taken per
+// device at qc_init (host code never uses the symbol itself).  Every call on
+// a GPU shares that GPU's copy, so calls that overlap on one GPU race.
+constexpr int64_t kSymvScratchBytes = int64_t(64) << 20;
+static __device__ __align__(256) double scratch[kSymvScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static void* symv_scratch_ptr(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 __host__ __device__ inline int64_t sm_tiles_per_dim(int64_t nbf) {
@@ -587,6 +606,18 @@ __global__ __launch_bounds__(256) void dense_jk_kernel(const double* __restrict_
   }
 }
 
+// Called by qc_init with `device` current.
+int symv_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return set_error("qc_init: device %d beyond the %d the scratch table holds", device, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  if (check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return 1;
+  g_scratch_ptr[device] = p;
+  return 0;
+}
+
 }  // namespace qcmd
 
 using namespace qcmd;
@@ -679,6 +710,22 @@ static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const d
 int qc_symv(const double* M, int64_t nbf, int nrhs, const double* x, double* y, void* work,
             void* stream) {
   return symv_phase(3, M, nbf, nrhs, x, y, work, stream);
+}
+
+int qc_symv_noalloc(const double* M, int64_t nbf, int nrhs, const double* x, double* y,
+                    void* stream) {
+  if (nrhs < 1 || nrhs > 4) return set_error("qc_symv_noalloc: nrhs %d outside [1, 4]", nrhs);
+  if (nbf <= 0) return set_error("qc_symv_noalloc: nbf %lld", (long long)nbf);
+  const int64_t need = symv_work_bytes(nbf, nrhs, g_symv_seg);
+  if (need > kSymvScratchBytes)
+    return set_error("qc_symv_noalloc: nbf %lld with %d right-hand sides needs %lld B of "
+                     "workspace, above the %lld B scratch (use qc_symv with a caller workspace)",
+                     (long long)nbf, nrhs, (long long)need, (long long)kSymvScratchBytes);
+  int dev = -1;
+  if (check(hipGetDevice(&dev), "qc_symv_noalloc: hipGetDevice")) return 1;
+  void* w = symv_scratch_ptr(dev);
+  if (!w) return set_error("qc_symv_noalloc: qc_init not called on device %d", dev);
+  return symv_phase(3, M, nbf, nrhs, x, y, w, stream);
 }
 
 int qc_symv_tiles(const double* M, int64_t nbf, int nrhs, const double* x, void* work,

@@ -9,6 +9,7 @@ int set_error(const char* fmt, ...);
 int check(hipError_t e, const char* what);
 const double* boys_table64();
 const float* boys_table32();
+int symv_scratch_init(int device);  // jk.hip: device address of the qc_symv_noalloc scratch
 
 // The 21 ERI classes (la lb | lc ld), la >= lb, lc >= ld, pair class
 // la(la+1)/2+lb >= lc(lc+1)/2+ld, l <= 2.

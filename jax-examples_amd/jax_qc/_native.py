@@ -67,6 +67,7 @@ _SIGS = {
                             c_void_p],
     "qc_symv_config": [c_int, c_int],
     "qc_symv_split": [c_int, c_int],
+    "qc_symv_noalloc": [c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p],
     "qc_block_maxabs": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
     "qc_symv_screened_tiles": [c_void_p, c_void_p, c_void_p, c_void_p, c_i64, c_int, c_void_p,
                                c_void_p, c_double, c_void_p, c_void_p],
