@@ -56,7 +56,7 @@ typedef struct {
   double* schwarz;        /* per shell pair bound (mode 2) */
   const double* boys64;   /* filled in by the library */
   const float* boys32;
-  int variant;            /* 0 built-in choice, 1 one thread per quartet, 2 one wave per quartet */
+  int variant;            /* 0 the per-class form measured faster on MI355X (one wave per quartet for (dp|dp), (dd|pp), (dd|dp), (dd|dd), one thread otherwise), 1 one thread per quartet, 2 one wave per quartet */
   const int64_t* sp_base; /* mode 1, optional: per shell pair, element offset of the pair's
                              supermatrix in out (many molecules in one launch: every
                              quartet's bra and ket pair belong to one molecule, whose AO

@@ -181,9 +181,13 @@ int qc_eri(int cab, int ccd, int fp32, const qc_eri_args* args_in, void* stream)
   if (a.sp_base && a.mode != ERI_SUPER) return set_error("qc_eri: sp_base needs mode 1");
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
+  // variant 0: the form measured faster on MI355X ((H2O)_8 even-tempered, FP64):
+  // one wave per quartet for (dp|dp), (dd|pp), (dd|dp), (dd|dd), one thread
+  // per quartet for every other class (e.g. (dp|pp) 14.9 vs 78 ms)
+  const bool wave_default = (cab == 4 && ccd == 4) || (cab == 5 && (ccd == 2 || ccd == 4 || ccd == 5));
 #define QC_CALL(A, B, C, D)                                                                    \
   if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D)) {                  \
-    const bool coop = a.variant == 2 || (a.variant == 0 && (A) + (B) + (C) + (D) >= 5);        \
+    const bool coop = a.variant == 2 || (a.variant == 0 && wave_default);                      \
     if (coop)                                                                                  \
       e = fp32 ? eri_launchc_##A##B##C##D##_f32(a, s) : eri_launchc_##A##B##C##D##_f64(a, s); \
     else                                                                                       \

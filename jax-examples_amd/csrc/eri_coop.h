@@ -209,7 +209,6 @@ __global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT)) void eri_coop_
   if (gid >= args.nquartets) return;  // whole work-group (one wave)
   locate(args, gid, bp, kp);
   const Real* __restrict__ pp = pair_table<Real>(args);
-  const Real* __restrict__ tab = boys_table<Real>(args);
   const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
   const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
 
@@ -280,7 +279,7 @@ __global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT)) void eri_coop_
         const Real X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
                    Z = P[PP_PX + 2] - Q[PP_PX + 2];
         Real F[L + 1];
-        boys_fn<L, Real>(alpha * (X * X + Y * Y + Z * Z), tab, F);
+        boys_eval<L, Real>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
         Real pw = (Real)1;
 #pragma unroll
         for (int n = 0; n <= L; ++n) {

@@ -86,7 +86,6 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
   locate(args, gid, bp, kp);
 
   const Real* __restrict__ pp = pair_table<Real>(args);
-  const Real* __restrict__ tab = boys_table<Real>(args);
   const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
   const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
 
@@ -121,7 +120,7 @@ __global__ __launch_bounds__(128) void eri_class_kernel(EriArgs args) {
       const Real X = P[PP_PX] - Qx, Y = P[PP_PX + 1] - Qy, Z = P[PP_PX + 2] - Qz;
       const Real T = alpha * (X * X + Y * Y + Z * Z);
       Real F[L + 1];
-      boys_fn<L, Real>(T, tab, F);
+      boys_eval<L, Real>(T, args.boys64, F);
       Real R[L + 1][L + 1][L + 1];
       hermite_r<L, Real>(R, F, (Real)-2 * alpha, X, Y, Z);
       const Real pref = (Real)34.986836655249725693 / (pq * dev_sqrt(ps)) * P[PP_K] * Kcd;

@@ -170,7 +170,7 @@ __device__ __forceinline__ bool grad2e_thread_quartet(const Grad2eArgs& args, in
       const Real X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
                    Z = P[PP_PX + 2] - Q[PP_PX + 2];
       Real F[L1 + 1];
-      boys_fn<L1, Real>(alpha * (X * X + Y * Y + Z * Z), gboys<Real>(args), F);
+      boys_eval<L1, Real>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
       Real R[L1 + 1][L1 + 1][L1 + 1];
       hermite_r<L1, Real>(R, F, (Real)-2 * alpha, X, Y, Z);
       const Real pref = (Real)34.986836655249725693 / (pqm * dev_sqrt(ps)) * P[PP_K] * Q[PP_K];
@@ -672,7 +672,7 @@ void grad2e_wave_kernel(Grad2eArgs args) {
           const Real X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
                      Z = P[PP_PX + 2] - Q[PP_PX + 2];
           Real F[L1 + 1];
-          boys_fn<L1, Real>(alpha * (X * X + Y * Y + Z * Z), gboys<Real>(args), F);
+          boys_eval<L1, Real>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
           Real pw = (Real)1;
 #pragma unroll
           for (int n = 0; n <= L1; ++n) {

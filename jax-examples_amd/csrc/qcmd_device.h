@@ -79,6 +79,24 @@ __device__ __forceinline__ void boys_fn(Real T, const Real* __restrict__ tab, Re
   }
 }
 
+// F[0..N] at T for a kernel computing in Real.  FP64 kernels evaluate the
+// FP64 grid directly.  FP32 kernels (the mixed path's primitive sums)
+// evaluate the same FP64 grid, Taylor series and recursion in FP64 and round
+// the values to FP32: the Boys function stays an FP64 evaluation, only the
+// primitive sums that consume it run in FP32.
+template <int N, typename Real>
+__device__ __forceinline__ void boys_eval(Real T, const double* __restrict__ tab64,
+                                          Real (&F)[N + 1]) {
+  if constexpr (sizeof(Real) == 8) {
+    boys_fn<N, double>(T, tab64, F);
+  } else {
+    double Fd[N + 1];
+    boys_fn<N, double>((double)T, tab64, Fd);
+#pragma unroll
+    for (int n = 0; n <= N; ++n) F[n] = (Real)Fd[n];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Component tables
 // ---------------------------------------------------------------------------

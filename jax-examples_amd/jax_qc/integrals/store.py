@@ -21,7 +21,7 @@ from __future__ import annotations
 
 import math
 import time
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 import torch
@@ -212,32 +212,60 @@ class ERIStore:
     def last_build_incremental(self) -> bool:
         return self.inc is not None and self.inc.last_incremental
 
-    def _inc_build(self, dens, full_fn, inc_fn):
+    def _inc_fock(self, H, dens, full_fn, inc_fn):
+        """Fock matrices H + G(dens) by a full or an incremental build (see
+        set_incremental); on the GPU each kind of build is one HIP graph
+        replay after its first (eager) use."""
         inc = self.inc
-        full = (inc.prev is None or inc.force_full or inc.final
+        full = (inc.G is None or inc.force_full or inc.final
                 or (inc.full_every > 0 and inc.since_full >= inc.full_every))
+        kind = "full" if full else "inc"
+        if inc.G is None:
+            k = len(dens)
+            z = torch.zeros((k,) + tuple(dens[0].shape), dtype=dens[0].dtype, device=dens[0].device)
+            inc.G, inc.prev, inc.Din, inc.F = z, z.clone(), z.clone(), z.clone()
+            inc.H = torch.empty_like(dens[0])
+        inc.Din.copy_(torch.stack(dens))
+        inc.H.copy_(H)
+
+        def body():
+            if kind == "full":
+                inc.G.copy_(torch.stack(full_fn(list(inc.Din))))
+            else:
+                inc.G.add_(torch.stack(inc_fn(list(inc.Din - inc.prev))))
+            inc.prev.copy_(inc.Din)
+            inc.F.copy_(inc.H + inc.G)
+
+        graphs = USE_FOCK_GRAPHS and inc.Din.is_cuda
+        g = inc.graphs.get(kind) if graphs else None
+        if g is not None:
+            g.replay()
+        elif graphs and kind in inc.warm:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                body()
+            inc.graphs[kind] = g
+            g.replay()
+        else:
+            body()  # first use of this kind (also the warm-up before capture)
+            inc.warm.add(kind)
         if full:
-            G = full_fn(dens)
             inc.since_full = 0
             inc.n_full += 1
         else:
-            dG = inc_fn([d - p for d, p in zip(dens, inc.prev)])
-            G = [g + dg for g, dg in zip(inc.G, dG)]
             inc.since_full += 1
             inc.n_inc += 1
         inc.force_full = False
         inc.last_incremental = not full
-        inc.prev = [d.clone() for d in dens]
-        inc.G = G
-        return G
+        return [f.clone() for f in inc.F]
 
     def fock_rhf(self, H, D):
         """F = H + J(D) - 1/2 K(D) from the P supermatrix."""
         if self.inc is not None:
             P = self._compact_or_raise(self.ensure_P())
-            G = self._inc_build([D], lambda d: [self.contract(P, d[0])],
-                                lambda d: [self.contract(P, d[0], tau_skip=self.inc.tau)])
-            return H + G[0]
+            F = self._inc_fock(H, [D], lambda d: [self.contract(P, d[0])],
+                               lambda d: [self.contract(P, d[0], tau_skip=self.inc.tau)])
+            return F[0]
         return self.contract(self.ensure_P(), D, H)
 
     def _compact_or_raise(self, M):
@@ -264,8 +292,8 @@ class ERIStore:
                 Gk = self.contract(K, torch.stack(d), tau_skip=tau)
                 return [Gj - Gk[0], Gj - Gk[1]]
 
-            G = self._inc_build([Da, Db], build, lambda d: build(d, self.inc.tau))
-            return H + G[0], H + G[1]
+            F = self._inc_fock(H, [Da, Db], build, lambda d: build(d, self.inc.tau))
+            return F[0], F[1]
         Jf = self.coulomb_plus(H, Da + Db)
         K = self.contract(self.ensure_K(), torch.stack([Da, Db]))
         return Jf - K[0], Jf - K[1]
@@ -276,12 +304,24 @@ class ERIStore:
         return self.contract(self.J, D, H)
 
 
+# Replay the incremental and full Fock builds of incremental mode from HIP
+# graphs after their first use (a build is ~25 launches of small kernels
+# around the two tile passes); off: every build is launched eagerly.
+USE_FOCK_GRAPHS = True
+
+
 @dataclass
 class _Incremental:
     tau: float
     full_every: int
-    prev: list | None = None
-    G: list | None = None
+    # device state: G(D_prev), D_prev, the input and output of a build, H
+    G: torch.Tensor | None = None
+    prev: torch.Tensor | None = None
+    Din: torch.Tensor | None = None
+    F: torch.Tensor | None = None
+    H: torch.Tensor | None = None
+    graphs: dict = field(default_factory=dict)
+    warm: set = field(default_factory=set)
     since_full: int = 0
     force_full: bool = False
     final: bool = False

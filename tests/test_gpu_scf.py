@@ -101,3 +101,38 @@ def test_water8_energy_rotation_invariant():
         del ints
         torch.cuda.empty_cache()
     assert E[0] == pytest.approx(E[1], abs=1e-8)
+
+
+def test_purification_graph_replays_match_eager():
+    """The graph-replayed TC2 iteration gives the eager iteration's
+    projectors on a sequence of Fock pairs (same shape, so the captured
+    graphs are replayed for each), with eager work between the calls, and
+    reports the same non-convergence for a spectrum without a gap."""
+    import numpy as np
+
+    from jax_qc.scf import purify
+
+    rng = np.random.default_rng(11)
+    n = 96
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    for rep in range(4):
+        Fs = []
+        for b in range(2):
+            w = np.sort(rng.uniform(-20, 5, n))
+            w[17 + b:] += 0.5  # a gap after 17 (alpha) / 18 (beta) occupied
+            Fs.append((Q * w[None, :]) @ Q.T)
+        F = torch.from_numpy(np.stack(Fs)).cuda()
+        try:
+            purify.USE_GRAPHS = True
+            Pg = purify.occupied_projectors(F, [17, 18])
+            _ = (F @ F).sum().item()  # eager work between replays
+            purify.USE_GRAPHS = False
+            Pe = purify.occupied_projectors(F, [17, 18])
+        finally:
+            purify.USE_GRAPHS = True
+        assert Pg is not None and Pe is not None
+        assert float((Pg - Pe).abs().max()) < 1e-13, rep
+    w = np.linspace(-10.0, 10.0, n)
+    w[5] = w[4]
+    F = torch.from_numpy((Q * w[None, :]) @ Q.T)[None].cuda()
+    assert purify.occupied_projectors(F, [5], max_iter=80) is None
