@@ -8,7 +8,8 @@ functions), energy + analytic gradient through two paths:
   FP64 ERIs, FP64 supermatrices, full Fock builds, FP64 gradient at 1e-12.
 * mixed: ERI primitive sums in FP32 for quartets with Schwarz bound < 1e-4,
   supermatrix tiles below max|element| 1e-2 stored in FP32, primitive pairs
-  with a Cauchy-Schwarz bound below 1e-12 left out, incremental Fock builds
+  with a Cauchy-Schwarz bound below 1e-12 left out (1e-10 for the derivative
+  integrals), incremental Fock builds
   (G(D) = G(D_prev) + G(D - D_prev), tiles screened at 1e-13, full builds
   at the start and for convergence), gradient with Schwarz 1e-11, density
   screening 1e-10 and FP32 primitive sums below bound 1e-2; FP64 Fock
@@ -53,7 +54,8 @@ HBM_PEAK_GBS = 8000.0
 # rest (2.6e-7 at 1e-10, 8.5e-7 at 3e-10)
 MIXED_GRAD = dict(grad_tau=1e-11, grad_tau_density=1e-10, grad_tau32=1e-2)
 # the whole mixed path (see the module docstring)
-MIXED = dict(tau=1e-12, tau32=1e-4, tile_f32=1e-2, prim_eps=1e-12, fock_tau=1e-13, **MIXED_GRAD)
+MIXED = dict(tau=1e-12, tau32=1e-4, tile_f32=1e-2, prim_eps=1e-12, fock_tau=1e-13,
+             grad_prim_eps=1e-10, **MIXED_GRAD)
 STRICT = dict(tau=1e-12, grad_tau=1e-12)
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
@@ -64,9 +66,11 @@ FP32_PEAK_TFLOPS = 157.3
 # x1.00); profiles/symv_tiles_water8_*.md hold the commands and counts.
 MEASURED_TRAFFIC = {"fp64": (1639021.6 * 2.0 + 54269.0) * 1024,
                     "mixed": (1152157.2 * 2.0 + 63845.3) * 1024}
-# the same for the K tile pass of the (H2O)_16 mixed workload (None: not yet
-# measured for the current pool layout)
-TRAFFIC_C5_K = None
+# the same for the K tile pass (symv_tiles_kernel<2,true,true,2>) of the
+# (H2O)_16 mixed workload: FETCH 12,877,282 KiB x 2.00 + WRITE 1,643,874 KiB
+# per launch (profiles/fock_symv_water16_uhf_mixed.md), 1.08 x the 26.0 GB
+# algorithmic pool bytes (the rest: column partials and the tile directory)
+TRAFFIC_C5_K = (12877282 * 2 + 1643874) * 1024
 
 
 def _cpu_info():

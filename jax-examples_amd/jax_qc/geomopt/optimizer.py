@@ -38,10 +38,12 @@ class OptResult(NamedTuple):
 def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12, tau32=None,
                         tile_f32=None, grad_tau=1e-13, grad_tau_density=0.0, grad_tau32=None,
                         prim_eps=None, fock_tau=None, fock_full_every=0, timings=None,
-                        keep_store=False):
+                        keep_store=False, grad_prim_eps=None):
     """Energy and gradient at ``coords``.  Mixed-path options: tau32 (FP32
     ERI sums below that Schwarz bound), tile_f32 (FP32 supermatrix tiles),
-    prim_eps (primitive-pair screening of the ERIs and derivative ERIs),
+    prim_eps (primitive-pair screening of the ERIs and derivative ERIs;
+    grad_prim_eps, when set, screens the derivative ERIs' primitive pairs at
+    its own level),
     fock_tau (incremental Fock builds with tile screening), grad_tau_density
     and grad_tau32 (gradient screening / FP32 sums).  ``timings`` (a dict)
     receives the wall time of each stage, the device synchronised between
@@ -72,8 +74,13 @@ def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12,
     t = mark("integrals", t)
     res = run_scf(m, ints, config)
     t = mark("scf", t)
-    g = scf_gradient(m, bs, res, grad_tau, pair_table=getattr(ints.ERI, "pt", None),
-                     tau_density=grad_tau_density, tau32=grad_tau32)
+    gpt = getattr(ints.ERI, "pt", None)
+    if grad_prim_eps is not None and gpt is not None:
+        from jax_qc.integrals.engine import screen_primitives
+
+        gpt = screen_primitives(getattr(gpt, "unscreened", gpt), grad_prim_eps)
+    g = scf_gradient(m, bs, res, grad_tau, pair_table=gpt, tau_density=grad_tau_density,
+                     tau32=grad_tau32)
     t = mark("gradient", t)
     if timings is not None and hasattr(ints.ERI, "inc") and ints.ERI.inc is not None:
         timings["fock_full_builds"] = ints.ERI.inc.n_full
