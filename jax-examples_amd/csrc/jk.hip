@@ -455,29 +455,52 @@ __global__ __launch_bounds__(256) void tile_pack_kernel(const double* __restrict
 }
 
 // y[h][R*64 + r] = sum of the row partials of row R's items plus the column
-// partials of tiles (R', R), R' > R; four waves split the terms.
-__global__ __launch_bounds__(256) void symv_reduce_kernel(int nrhs, int64_t T, int64_t npad,
-                                                          const double* __restrict__ rowbuf,
-                                                          const double* __restrict__ colbuf,
-                                                          double* __restrict__ y, int seg,
-                                                          const uint8_t* __restrict__ fmt,
-                                                          TileScreen scr) {
+// partials of tiles (R', R), R' > R.  RW waves split the terms, each keeping
+// four independent sums so that four partial-block loads are in flight per
+// wave (the column partials of one row are 512-byte blocks far apart in the
+// workspace: one load at a time per wave left the reduce latency-bound).
+constexpr int RW = 8;
+
+__device__ __forceinline__ bool reduce_live(const uint8_t* __restrict__ fmt,
+                                            const TileScreen& scr, int64_t tile, int64_t Rp,
+                                            float xmR) {
+  return !scr.tmax || tile_live(fmt, scr, tile, scr.xmax[Rp], xmR);
+}
+
+__global__ __launch_bounds__(64 * RW) void symv_reduce_kernel(int nrhs, int64_t T, int64_t npad,
+                                                              const double* __restrict__ rowbuf,
+                                                              const double* __restrict__ colbuf,
+                                                              double* __restrict__ y, int seg,
+                                                              const uint8_t* __restrict__ fmt,
+                                                              TileScreen scr) {
   const int64_t R = blockIdx.x;
   const int h = blockIdx.y;
   const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double acc = 0.0;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   const int64_t i0 = item_off(R, seg), i1 = item_off(R + 1, seg);
-  for (int64_t it = i0 + w; it < i1; it += 4) acc += rowbuf[(it * nrhs + h) * SM_B + r];
+  for (int64_t it = i0 + w; it < i1; it += RW) a0 += rowbuf[(it * nrhs + h) * SM_B + r];
   const float xmR = scr.tmax ? scr.xmax[R] : 0.0f;
-  for (int64_t Rp = R + 1 + w; Rp < T; Rp += 4) {
+  auto col = [&](int64_t Rp) -> double {
     const int64_t tile = Rp * (Rp + 1) / 2 + R;
-    if (scr.tmax && !tile_live(fmt, scr, tile, scr.xmax[Rp], xmR)) continue;
-    acc += colbuf[(tile * nrhs + h) * SM_B + r];
+    return reduce_live(fmt, scr, tile, Rp, xmR) ? colbuf[(tile * nrhs + h) * SM_B + r] : 0.0;
+  };
+  int64_t Rp = R + 1 + w;
+  for (; Rp + 3 * RW < T; Rp += 4 * RW) {
+    a0 += col(Rp);
+    a1 += col(Rp + RW);
+    a2 += col(Rp + 2 * RW);
+    a3 += col(Rp + 3 * RW);
   }
-  __shared__ double part[4][64];
-  part[w][r] = acc;
+  for (; Rp < T; Rp += RW) a0 += col(Rp);
+  __shared__ double part[RW][64];
+  part[w][r] = (a0 + a1) + (a2 + a3);
   __syncthreads();
-  if (w == 0) y[h * npad + R * SM_B + r] = part[0][r] + part[1][r] + part[2][r] + part[3][r];
+  if (w == 0) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < RW; ++k) v += part[k][r];
+    y[h * npad + R * SM_B + r] = v;
+  }
 }
 
 // Batched reduce: one block per global packed row g (all molecules).
@@ -700,7 +723,7 @@ static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const d
     if (check(hipGetLastError(), "qc_symv tiles")) return 1;
   }
   if (phase & 2) {
-    hipLaunchKernelGGL(symv_reduce_kernel, dim3((unsigned)T, nrhs), dim3(256), 0, s, nrhs, T,
+    hipLaunchKernelGGL(symv_reduce_kernel, dim3((unsigned)T, nrhs), dim3(64 * RW), 0, s, nrhs, T,
                        npad, rowbuf, colbuf, y, seg, fmt, scr);
     if (check(hipGetLastError(), "qc_symv reduce")) return 1;
   }
