@@ -529,6 +529,69 @@ def profile_water16(args):
                       "K_bytes": nbytes["K"]}), flush=True)
 
 
+def grad_report(args):
+    """Per-launch derivative-ERI kernel times (HIP events around each
+    qc_grad2e call) and algorithmic FLOP rates on the (H2O)_16 gradient of
+    both paths; one JSON line per path.  FLOP = sum over the launch's
+    quartets of (bra primitive pairs x ket primitive pairs) x
+    engine.grad_flops_per_primitive_quartet (mean ket primitive count)."""
+    import numpy as np
+    import torch
+
+    from jax_qc import molecules as M
+    from jax_qc.core.types import CalcConfig
+    from jax_qc.geomopt.optimizer import energy_and_gradient
+    from jax_qc.grad import analytic as an
+    from jax_qc.integrals import engine as eng
+
+    torch.cuda.set_device(0)
+    mol = M.water_cluster((4, 2, 2), seed=2, spin=2)
+    cfgc = CalcConfig(method="uhf", basis="even-tempered", scf_conv=1e-10, max_scf_iter=200)
+    for mode, opts in (("fp64", STRICT), ("mixed", MIXED)):
+        E, g, m, res = energy_and_gradient(np.asarray(mol.coords), mol, "even-tempered", cfgc,
+                                           keep_store=True, **opts)
+        pt = res.ERI.pt
+        if opts.get("grad_prim_eps"):
+            pt = eng.screen_primitives(getattr(pt, "unscreened", pt), opts["grad_prim_eps"])
+        Dj, Dk, cx, W = an._densities(m, res)
+        rep = []
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        an.electronic_gradient(m, res.ERI.basis, Dj, Dk, cx, W, opts["grad_tau"], pt,
+                               parts=("2e",), tau_density=opts.get("grad_tau_density", 0.0),
+                               tau32=opts.get("grad_tau32"), report=rep)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        npp = pt.sp_nprim.double()
+        rows, tot_fl, tot_ms = [], 0.0, 0.0
+        for r in rep:
+            ms = r["events"][0].elapsed_time(r["events"][1])
+            cnt = (r["offsets"][1:] - r["offsets"][:-1])
+            cum = torch.cat([torch.zeros(1, dtype=torch.float64, device=npp.device),
+                             torch.cumsum(npp[r["ket"].long()], 0)])
+            ks = r["kstart"].long()
+            per_bra = cum[ks + cnt] - cum[ks]
+            nprimq = float((npp[r["bra"].long()] * per_bra).sum())
+            mean_k = float(per_bra.sum() / max(1, int(cnt.sum()))) if int(cnt.sum()) else 0.0
+            la, lb = eng.CLASS_L[r["cab"]]
+            lc, ld = eng.CLASS_L[r["ccd"]]
+            fl = nprimq * eng.grad_flops_per_primitive_quartet(la, lb, lc, ld, mean_k)
+            peak = FP32_PEAK_TFLOPS if r["fp32"] else FP64_PEAK_TFLOPS
+            rows.append({"class": [la, lb, lc, ld], "fp32": r["fp32"], "quartets": r["nquartets"],
+                         "primitive_quartets": round(nprimq), "ms": round(ms, 3),
+                         "tflops": round(fl / (ms * 1e-3) / 1e12, 3) if ms > 0 else None,
+                         "frac_of_peak": round(fl / (ms * 1e-3) / 1e12 / peak, 4) if ms > 0 else None})
+            tot_fl += fl
+            tot_ms += ms
+        rows.sort(key=lambda x: -x["ms"])
+        print(json.dumps({"grad_report": mode, "wall_s": round(wall, 4), "kernel_ms": round(tot_ms, 2),
+                          "tflop": round(tot_fl / 1e12, 4),
+                          "tflops": round(tot_fl / (tot_ms * 1e-3) / 1e12, 3), "launches": rows}),
+              flush=True)
+        del res
+        torch.cuda.empty_cache()
+
+
 def run_water16(args):
     """(H2O)_16 UHF triplet energy + gradient, FP64 vs mixed; mixed Fock
     builds/s as the headline value (see the module docstring)."""
@@ -692,6 +755,9 @@ def main():
                     help="also print per-class ERI kernel times and FLOP rates (one JSON line "
                          "per precision mode)")
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
+    ap.add_argument("--grad-report", action="store_true",
+                    help="water16: per-launch derivative-ERI kernel times and FLOP rates of "
+                         "both paths' gradients (JSON lines; no benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the quartet "
@@ -700,6 +766,8 @@ def main():
     if args.workload == "batch":
         return run_batch(args)
     if args.workload in ("water16", "water16-uhf"):
+        if args.grad_report:
+            return grad_report(args)
         if args.profile_steps:
             return profile_water16(args)
         return run_water16(args)

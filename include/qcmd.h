@@ -237,7 +237,10 @@ typedef struct {
   double* grad;        /* natom x 3, accumulated (atomic adds) */
   int variant;         /* 0 library default, 1 one thread per quartet (low
                           classes only), 2 one wave per quartet with the bra
-                          side contracted with G first */
+                          side contracted with G first, 3 one wave per
+                          quartet with one lane per primitive quartet
+                          (classes with la+lb+lc+ld <= 6 except those with
+                          a thread kernel) */
   const double* schwarz; /* optional (with tau_density > 0): per shell pair Q */
   double tau_density;    /* skip a quartet when max|G_abcd| Q_ab Q_cd < tau_density */
   const float* pairs32;  /* fp32 = 1: FP32 primitive-pair table */

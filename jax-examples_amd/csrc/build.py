@@ -40,6 +40,13 @@ GRAD_PER_THREAD = {"0000", "1000", "1010", "1100", "1110", "2000", "2010", "2100
 # (ds|pp) keeps arrays on the stack, (dp|ps) and (dp|ds) hold one wave per
 # SIMD at ~400 registers, and those lose: 760 vs 154 ms for (dp|ps))
 GRAD_PER_THREAD_F32 = GRAD_PER_THREAD | {"2200"}
+# derivative-ERI classes with the one-lane-per-primitive-quartet kernel
+# (grad_prim.h, one launch per ket Cartesian pair): those whose R and h lists
+# fit a lane's registers without spills (FP64 (ds|pp) 239 VGPRs, (pp|pp) 256;
+# FP32 (ds|pp) 134, (dp|pp) 233, (dp|ds) 229; FP64 (dp|ps) and (dd|ss) and
+# FP32 (dd|ps) spill 100-700 registers)
+GRAD_PRIM = {"1111", "2011"}
+GRAD_PRIM_F32 = GRAD_PRIM | {"2110", "2111", "2120"}
 STATIC_SOURCES = ["capi.hip", "oneint.hip", "jk.hip", "grad.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -75,7 +82,7 @@ def _generate():
             # rough cost for scheduling: total angular momentum, fp64 first
             out.append((sum(c) * 2 + (prec == "f64"), path))
         path = os.path.join(GEN, f"grad2e_{tag}.hip")
-        text = ('#include "grad_impl.h"\n'
+        text = ('#include "grad_prim.h"\n'
                 "namespace qcmd {\n"
                 f"hipError_t grad2e_launchw_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
@@ -90,6 +97,14 @@ def _generate():
                 f"hipError_t grad2e_launcht32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float>(a, s);\n"
                    if tag in GRAD_PER_THREAD_F32 else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
+                + "}\n"
+                f"hipError_t grad2e_launchp_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                + (f"  return grad2e_launch_prim<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
+                   if tag in GRAD_PRIM else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
+                + "}\n"
+                f"hipError_t grad2e_launchp32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                + (f"  return grad2e_launch_prim<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float>(a, s);\n"
+                   if tag in GRAD_PRIM_F32 else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
                 + "}\n}  // namespace qcmd\n")
         if not os.path.exists(path) or open(path).read() != text:
             with open(path, "w") as fh:

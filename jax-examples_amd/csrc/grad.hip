@@ -182,7 +182,9 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
   hipError_t grad2e_launcht_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);     \
   hipError_t grad2e_launcht32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);   \
   hipError_t grad2e_launchw_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);   \
-  hipError_t grad2e_launchw32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
+  hipError_t grad2e_launchw32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);  \
+  hipError_t grad2e_launchp_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);    \
+  hipError_t grad2e_launchp32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
 QC_ERI_CLASSES(QC_GDECL)
 #undef QC_GDECL
 
@@ -256,7 +258,7 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   if (a.tau_density > 0.0 && !a.schwarz) return set_error("qc_grad2e: tau_density without schwarz");
   a.boys64 = boys_table64();
   if (!a.boys64) return set_error("qc_grad2e: qc_init not called on the current device");
-  if (in->variant < 0 || in->variant > 2) return set_error("qc_grad2e: variant %d", in->variant);
+  if (in->variant < 0 || in->variant > 3) return set_error("qc_grad2e: variant %d", in->variant);
   // default (measured on (H2O)_8 and (H2O)_16): one thread per quartet for
   // every class that has that kernel (ket (ss) up to (dp|ss), ket (ps) up to
   // (ds|ps), and (ds|ds)), one wave with the bra side contracted first for
@@ -264,17 +266,27 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   const bool has_thread = (ccd == 0 && cab <= 4) || (ccd == 1 && cab <= 3) || (cab == 3 && ccd == 3);
   // in FP32 the thread kernel also fits (dd|ss)
   const bool has_thread32 = has_thread || (cab == 5 && ccd == 0);
-  const int var = in->variant != 0 ? in->variant : ((in->fp32 ? has_thread32 : has_thread) ? 1 : 2);
+  // one lane per primitive quartet where that form measured faster on
+  // (H2O)_16 (wave form -> per-primitive form): FP64 (pp|pp) 773 -> 453 ms,
+  // (ds|pp) 1183 -> 711 ms; FP32 (pp|pp) 121 -> 104, (dp|ps) 123 -> 106,
+  // (dp|pp) 111 -> 94, (dp|ds) 56 -> 41 ms (FP32 (ds|pp) 142 vs 143: wave)
+  const bool prim64 = (cab == 2 && ccd == 2) || (cab == 3 && ccd == 2);
+  const bool prim32 = (cab == 2 && ccd == 2) || (cab == 4 && ccd >= 1 && ccd <= 3);
+  const int var = in->variant != 0
+                      ? in->variant
+                      : ((in->fp32 ? has_thread32 : has_thread) ? 1 : ((in->fp32 ? prim32 : prim64) ? 3 : 2));
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
 #define QC_GC(A, B, C, D)                                                                     \
   if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D))                    \
     e = var == 1 ? (in->fp32 ? grad2e_launcht32_##A##B##C##D(a, s) : grad2e_launcht_##A##B##C##D(a, s))  \
+      : var == 3 ? (in->fp32 ? grad2e_launchp32_##A##B##C##D(a, s) : grad2e_launchp_##A##B##C##D(a, s))  \
                  : (in->fp32 ? grad2e_launchw32_##A##B##C##D(a, s) : grad2e_launchw_##A##B##C##D(a, s));
   QC_ERI_CLASSES(QC_GC)
 #undef QC_GC
   if (e == hipErrorNotSupported)
-    return set_error("qc_grad2e: class (%d, %d) has no one-thread-per-quartet kernel", cab, ccd);
+    return set_error("qc_grad2e: class (%d, %d) has no %s kernel", cab, ccd,
+                     var == 3 ? "one-lane-per-primitive-quartet" : "one-thread-per-quartet");
   return check(e, "qc_grad2e launch");
 }
 

@@ -35,15 +35,24 @@ GRAD_VARIANT_OVERRIDE = None
 GRAD_STAGE_IN_LDS = True
 
 
-def grad_variant_for(cab: int, ccd: int) -> int:
+def grad_variant_for(cab: int, ccd: int, fp32: int = 0) -> int:
     if GRAD_VARIANT_OVERRIDE is not None:
-        return GRAD_VARIANT_OVERRIDE if (GRAD_VARIANT_OVERRIDE != 1 or
-                                         (cab, ccd) in THREAD_CAPABLE) else 2
-    return GRAD_VARIANTS.get((cab, ccd), 0)
+        ov = GRAD_VARIANT_OVERRIDE
+        if ov == 1 and (cab, ccd) not in THREAD_CAPABLE:
+            return 2
+        if ov == 3 and (cab, ccd) not in (PRIM_CAPABLE_F32 if fp32 else PRIM_CAPABLE):
+            return 2
+        return ov
+    return GRAD_VARIANTS.get((cab, ccd, int(fp32)), GRAD_VARIANTS.get((cab, ccd), 0))
 
 
 # classes with a one-thread-per-quartet kernel (as built by csrc/build.py)
 THREAD_CAPABLE = {(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1), (4, 0), (3, 3)}
+# classes with the one-lane-per-primitive-quartet kernel (csrc/build.py
+# GRAD_PRIM, GRAD_PRIM_F32): (pp|pp), (ds|pp); in FP32 also (dp|ps), (dp|pp),
+# (dp|ds)
+PRIM_CAPABLE = {(2, 2), (3, 2)}
+PRIM_CAPABLE_F32 = PRIM_CAPABLE | {(4, 1), (4, 2), (4, 3)}
 
 
 def energy_weighted_density(C, orbital_energies, n_occ: int, factor: float = 2.0):
@@ -126,7 +135,7 @@ def prescreen_launch(L, pt, shells, Mj, Mk, cx, Qd, tau_density):
 
 def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_table=None,
                         parts=("1e", "2e"), tau_density: float = 0.0,
-                        tau32: float | None = None) -> torch.Tensor:
+                        tau32: float | None = None, report: list | None = None) -> torch.Tensor:
     """sum D d(T+V) - sum W dS + 1/2 sum G dERI on the device, (natom, 3).
     tau: Schwarz threshold of the quartet list; tau_density > 0 also skips
     quartets with max|G_abcd| Q_ab Q_cd below it (density-weighted); tau32:
@@ -194,14 +203,23 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
             a.cx = cx
             a.grad = grad.data_ptr()
             a.natom = nat_ if GRAD_STAGE_IN_LDS else 0
-            a.variant = grad_variant_for(L.cab, L.ccd)
+            a.variant = grad_variant_for(L.cab, L.ccd, L.fp32)
             a.fp32 = L.fp32
             if L.fp32:
                 a.pairs32 = pt.pairs32.data_ptr()
             if Qd is not None:
                 a.schwarz = Qd.data_ptr()
                 a.tau_density = tau_density
+            if report is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
             nat.call("qc_grad2e", L.cab, L.ccd, ctypes.byref(a), s)
+            if report is not None:
+                e1.record()
+                report.append(dict(cab=L.cab, ccd=L.ccd, fp32=L.fp32, nquartets=L.nquartets,
+                                   variant=a.variant, events=(e0, e1), bra=L.bra, ket=L.ket,
+                                   offsets=L.offsets, kstart=L.kstart))
     return grad.reshape(nat_, 3)
 
 

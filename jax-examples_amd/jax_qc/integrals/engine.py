@@ -386,6 +386,28 @@ def flops_per_primitive_quartet(la, lb, lc, ld) -> int:
             + 2 * _nc(la) * _nc(lb) * _nh(Lab) * _nc(lc) * _nc(ld))
 
 
+def grad_flops_per_primitive_quartet(la, lb, lc, ld, ket_prims: float = 9.0) -> float:
+    """Algorithmic FLOP of one primitive quartet of the derivative-ERI
+    contraction (dA, dP = d(A+B), dC of sum G d(ab|cd)), in the form the
+    kernels evaluate it:
+    Boys (40 + 4(L+1)) + prefactor 30 + R to order L+1
+    (3(L+2)(L+3)(L+4)(L+5)/24) + h for t+u+v <= Lab+1 (2 N_H(Lab+1) N_H(Lcd)
+    nc_cd) + the three C-derivative lists for t+u+v <= Lab (6 N_H(Lab)
+    N_H(Lcd+1) nc_cd) + their contraction with Y (2 nc_cd (3 N_H(Lab+1) +
+    6 N_H(Lab))) + the bra-side Y = G E tables (8 nc_ab nc_cd N_H(Lab+1) per
+    bra primitive, shared by the quartet's ket primitives: divided by
+    ket_prims)."""
+    if la + lb < lc + ld:
+        la, lb, lc, ld = lc, ld, la, lb
+    L = la + lb + lc + ld
+    Lab, Lcd = la + lb, lc + ld
+    ncd, nab = _nc(lc) * _nc(ld), _nc(la) * _nc(lb)
+    return ((40 + 4 * (L + 1)) + 30 + 3 * (L + 2) * (L + 3) * (L + 4) * (L + 5) / 24
+            + 2 * _nh(Lab + 1) * _nh(Lcd) * ncd + 6 * _nh(Lab) * _nh(Lcd + 1) * ncd
+            + 2 * ncd * (3 * _nh(Lab + 1) + 6 * _nh(Lab))
+            + 8 * nab * ncd * _nh(Lab + 1) / max(ket_prims, 1.0))
+
+
 def plan_flops(pt: PairTable, plan: QuartetPlan, nprim_quartet: float | None = None) -> Dict[int, float]:
     """Algorithmic FLOPs of the plan split by precision (fp32 flag), using
     the mean primitive-pair count of each pair class."""
