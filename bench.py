@@ -47,16 +47,10 @@ for _p in (REPO, os.path.join(REPO, "jax-examples_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
-# gradient of the mixed path: Schwarz 1e-11, density-weighted screening 1e-10,
-# primitive sums in FP32 for quartets with Schwarz bound below 1e-2 (the FP64
-# path: Schwarz 1e-12, nothing else).  Measured on (H2O)_16: the FP32 bound
-# moves the gradient by 3e-8 Eh/Bohr at 1e-2; the density screening sets the
-# rest (2.6e-7 at 1e-10, 8.5e-7 at 3e-10)
-MIXED_GRAD = dict(grad_tau=1e-11, grad_tau_density=1e-10, grad_tau32=1e-2)
-# the whole mixed path (see the module docstring)
-MIXED = dict(tau=1e-12, tau32=1e-4, tile_f32=1e-2, prim_eps=1e-12, fock_tau=1e-13,
-             grad_prim_eps=1e-10, **MIXED_GRAD)
-STRICT = dict(tau=1e-12, grad_tau=1e-12)
+# the two precision modes (jax_qc/precision.py: the option sets and what each
+# screen / FP32 bound costs in accuracy on (H2O)_16)
+from jax_qc.precision import MIXED, MIXED_GRAD, STRICT  # noqa: E402
+
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 

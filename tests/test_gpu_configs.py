@@ -214,8 +214,9 @@ def _rigid_motion(coords, seed=7):
 
 
 def test_c5_water16_uhf_mixed_vs_fp64():
-    """(H2O)_16 UHF triplet, 480 functions: the mixed path's energy and
-    gradient against the FP64 path (1 uEh, 1e-6 Eh/Bohr); the gradient sums
+    """(H2O)_16 UHF triplet, 480 functions, in the benchmark's two modes
+    (jax_qc.precision): the mixed path's energy and gradient against the FP64
+    path (1 uEh, 1e-6 Eh/Bohr); the gradient sums
     to zero over atoms; the FP64 energy is unchanged by a rigid rotation +
     translation of the cluster."""
     from jax_qc.geomopt.optimizer import energy_and_gradient
@@ -225,9 +226,9 @@ def test_c5_water16_uhf_mixed_vs_fp64():
     mol = M.water_cluster((4, 2, 2), seed=2, spin=2)
     cfg = CalcConfig(method="uhf", basis="even-tempered", scf_conv=1e-10, max_scf_iter=200)
     c0 = np.asarray(mol.coords, dtype=np.float64)
-    strict = dict(tau=1e-12, grad_tau=1e-12)
-    mixed = dict(tau=1e-12, tau32=1e-4, tile_f32=1e-2, grad_tau=1e-11, grad_tau_density=1e-10,
-                 grad_tau32=1e-2)
+    # the benchmark's own option sets (FP32-split ERIs, FP32 tiles, primitive-pair
+    # screening, incremental Fock builds, screened FP32-split gradient)
+    from jax_qc.precision import MIXED as mixed, STRICT as strict
     E64, g64, _, r64 = energy_and_gradient(c0, mol, "even-tempered", cfg, **strict)
     Em, gm, _, rm = energy_and_gradient(c0, mol, "even-tempered", cfg, **mixed)
     assert r64.converged and rm.converged
