@@ -7,7 +7,8 @@ functions), energy + analytic gradient through two paths:
 * FP64 ("plain", the drop-in): Schwarz screening 1e-12 and nothing else;
   FP64 ERIs, FP64 supermatrices, full Fock builds, FP64 gradient at 1e-12.
 * mixed: ERI primitive sums in FP32 for quartets with Schwarz bound < 1e-4,
-  supermatrix tiles below max|element| 1e-2 stored in FP32, primitive pairs
+  supermatrix tiles below max|element| 1e-7 / 3e-5 / 1e-2 stored as int8 /
+  int16 (per-tile scale) / FP32, primitive pairs
   with a Cauchy-Schwarz bound below 1e-12 left out (1e-10 for the derivative
   integrals), incremental Fock builds
   (G(D) = G(D_prev) + G(D - D_prev), tiles screened at 1e-13, full builds
@@ -511,7 +512,7 @@ def profile_water16(args):
                                     prim_eps=MIXED["prim_eps"])
     st = ints.ERI
     st.ensure_K()
-    st.use_compact(tau_f32=args.tile_f32)
+    st.use_compact(tau_f32=args.tile_f32, tau_i16=MIXED["tile_i16"], tau_i8=MIXED["tile_i8"])
     n = bs.n_basis
     g = torch.Generator(device="cpu").manual_seed(0)
     A = torch.randn(2, n, n, generator=g, dtype=torch.float64) * 0.01
@@ -631,7 +632,8 @@ def run_water16(args):
                  n64=plan.n64, n32=plan.n32, eri_s=store.eri_seconds,
                  prim_pairs_kept=getattr(store.pt, "prim_kept", 1.0))
         if mode == "mixed":
-            P["tiles"] = {k: dict(zero=Mx.n_zero, f32=Mx.n_f32, f64=Mx.n_f64, bytes=Mx.bytes)
+            P["tiles"] = {k: dict(zero=Mx.n_zero, i8=Mx.n_i8, i16=Mx.n_i16, f32=Mx.n_f32,
+                                  f64=Mx.n_f64, bytes=Mx.bytes)
                           for k, Mx in (("J", store.J), ("K", store.K))}
             if not args.no_cpu:
                 P["cpu_eri"] = cpu_eri_build(store.basis, store.pt.unscreened

@@ -125,8 +125,12 @@ int qc_symv_reduce(int64_t nbf, int nrhs, double* y, void* work, void* stream);
 
 /* Mixed-precision supermatrix: per-tile max |element| (tmax[T(T+1)/2]),
  * packing into a pool under a caller-built directory (fmt[t] = 0 skip,
- * 1 FP32, 2 FP64; off[t] = byte offset in the pool), and the tile pass of the
- * Fock build over that pool (accumulation in FP64; qc_symv_reduce finishes). */
+ * 1 FP32 (16384 B), 2 FP64 (32768 B), 3 int16 (16 + 8192 B), 4 int8
+ * (16 + 4096 B); off[t] = byte offset in the pool, a multiple of 16), and the
+ * tile pass of the Fock build over that pool (accumulation in FP64;
+ * qc_symv_reduce finishes).  An int16 / int8 tile is an FP64 scale
+ * s = max|x| / 32767 (/ 127) and 8 bytes of padding, then the 64 x 64
+ * integers rint(x / s) row-major; qc_sm_pack computes s itself. */
 int qc_sm_tile_maxabs(const double* M, int64_t nbf, double* tmax, void* stream);
 int qc_sm_pack(const double* M, int64_t nbf, const uint8_t* fmt, const int64_t* off, void* pool,
                void* stream);

@@ -119,9 +119,17 @@ static int64_t symv_work_bytes(int64_t nbf, int nrhs, int seg) {
   return (items + tiles) * nrhs * SM_B * (int64_t)sizeof(double);
 }
 
-// Tile formats of a compacted (mixed-precision) supermatrix.
-enum TileFmt : uint8_t { TILE_ZERO = 0, TILE_F32 = 1, TILE_F64 = 2 };
+// Tile formats of a compacted (mixed-precision) supermatrix.  The two
+// integer formats hold q = rint(x / s) per element with one FP64 scale s =
+// max|x| / 32767 (int16) or max|x| / 127 (int8) per tile, so an element is
+// off by at most s / 2 = max|x| / 65534 or max|x| / 254: a fixed fraction of
+// the tile's largest element (an FP32 tile's error is relative to each
+// element instead).  Their pool block is a 16-byte header (s, then padding)
+// followed by the 64 x 64 integers, row-major.
+enum TileFmt : uint8_t { TILE_ZERO = 0, TILE_F32 = 1, TILE_F64 = 2, TILE_I16 = 3, TILE_I8 = 4 };
+constexpr int TILE_QHDR = 16;  // bytes before the integers of an int16 / int8 tile
 typedef float fvec2 __attribute__((ext_vector_type(2)));
+typedef int ivec2 __attribute__((ext_vector_type(2)));
 
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 
@@ -140,12 +148,12 @@ constexpr int SYMV_ROWS = 16;  // rows per lane
 #define SYMV_MIN_WAVES 1  // waves per SIMD the register budget must allow
 #endif
 
-// Column j < 4 of lane cg: FP32 tiles 4cg + j (16 B per row); FP64 tiles
-// 2cg + j for j < 2 and 32 + 2cg + (j-2) otherwise, so each 16-B load of a
-// wave covers 256 contiguous bytes of four rows.
+// Column j < 4 of lane cg: FP32 / int16 / int8 tiles 4cg + j (16 / 8 / 4 B
+// per row); FP64 tiles 2cg + j for j < 2 and 32 + 2cg + (j-2) otherwise, so
+// each 16-B load of a wave covers 256 contiguous bytes of four rows.
 template <typename T>
 __device__ __forceinline__ int lane_col(int cg, int j) {
-  if (sizeof(T) == 4) return 4 * cg + j;
+  if (sizeof(T) != 8) return 4 * cg + j;
   return j < 2 ? 2 * cg + j : 32 + 2 * cg + (j - 2);
 }
 
@@ -199,15 +207,18 @@ __device__ __forceinline__ void store_cols(double (&col)[NRHS][4], int64_t tile,
 }
 
 // One tile: lane (ro, cg) of wave `wave` covers rows r0 + 4k + ro (k < ROWS,
-// r0 = 64 wave / WS) and the four columns lane_col(cg, j).
+// r0 = 64 wave / WS) and the four columns lane_col(cg, j).  T = int16_t /
+// int8_t: the integers are summed against x scaled by the tile's scale sc,
+// and the column sums are scaled once at the end (sc is ignored otherwise).
 template <int NRHS, int WS, bool NT, typename T>
 __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile, bool diag,
                                           const double (&xr)[NRHS][SYMV_ROWS / WS],
                                           double (&rowacc)[NRHS][SYMV_ROWS / WS],
                                           const double* __restrict__ xcb, int64_t npad, int cg,
                                           int ro, int wave, int par, ColShare<NRHS, WS>& sh,
-                                          double* __restrict__ colbuf) {
+                                          double* __restrict__ colbuf, double sc = 1.0) {
   constexpr int ROWS = SYMV_ROWS / WS;
+  constexpr bool QUANT = sizeof(T) <= 2;
   const int r0 = (SM_B / WS) * wave;
   double xc[NRHS][4], col[NRHS][4];
 #pragma unroll
@@ -215,25 +226,39 @@ __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       xc[h][j] = xcb[h * npad + lane_col<T>(cg, j)];
+      if (QUANT) xc[h][j] *= sc;
       col[h][j] = 0.0;
     }
 #pragma unroll
   for (int k = 0; k < ROWS; ++k) {
     const T* row = tp + (r0 + 4 * k + ro) * SM_B;
     double m[4];
-    if (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 8) {
       const dvec2 a = ld<NT>(reinterpret_cast<const dvec2*>(row) + cg);
       const dvec2 b = ld<NT>(reinterpret_cast<const dvec2*>(row) + 16 + cg);
       m[0] = a.x;
       m[1] = a.y;
       m[2] = b.x;
       m[3] = b.y;
-    } else {
+    } else if constexpr (sizeof(T) == 4) {
       const fvec4 f = ld<NT>(reinterpret_cast<const fvec4*>(row) + cg);
       m[0] = f.x;
       m[1] = f.y;
       m[2] = f.z;
       m[3] = f.w;
+    } else if constexpr (sizeof(T) == 2) {
+      // columns 4cg .. 4cg+3: two dwords, low half first (little endian)
+      const ivec2 q = ld<NT>(reinterpret_cast<const ivec2*>(row) + cg);
+      m[0] = (double)(int16_t)q.x;
+      m[1] = (double)(int16_t)(q.x >> 16);
+      m[2] = (double)(int16_t)q.y;
+      m[3] = (double)(int16_t)(q.y >> 16);
+    } else {
+      const int q = ld<NT>(reinterpret_cast<const int*>(row) + cg);
+      m[0] = (double)(int8_t)q;
+      m[1] = (double)(int8_t)(q >> 8);
+      m[2] = (double)(int8_t)(q >> 16);
+      m[3] = (double)(int8_t)(q >> 24);
     }
 #pragma unroll
     for (int h = 0; h < NRHS; ++h) {
@@ -242,7 +267,15 @@ __device__ __forceinline__ void symv_tile(const T* __restrict__ tp, int64_t tile
       for (int j = 0; j < 4; ++j) col[h][j] += m[j] * xr[h][k];
     }
   }
-  if (!diag) store_cols<NRHS, WS, T>(col, tile, cg, ro, wave, par, sh, colbuf);
+  if (!diag) {
+    if (QUANT) {
+#pragma unroll
+      for (int h = 0; h < NRHS; ++h)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) col[h][j] *= sc;
+    }
+    store_cols<NRHS, WS, T>(col, tile, cg, ro, wave, par, sh, colbuf);
+  }
 }
 
 // Optional screen of a compacted supermatrix's tiles: a tile (R, C) is
@@ -342,6 +375,16 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
       } else if (f == TILE_F32) {
         symv_tile<NRHS, WS, NT, float>(reinterpret_cast<const float*>(base), tile, diag, xr,
                                        rowacc, xcb, npad, cg, ro, wave, par, sh, colbuf);
+      } else if (f == TILE_I16) {
+        const double sc = *reinterpret_cast<const double*>(base);  // uniform: one scalar load
+        symv_tile<NRHS, WS, NT, int16_t>(reinterpret_cast<const int16_t*>(base + TILE_QHDR), tile,
+                                         diag, xr, rowacc, xcb, npad, cg, ro, wave, par, sh,
+                                         colbuf, sc);
+      } else if (f == TILE_I8) {
+        const double sc = *reinterpret_cast<const double*>(base);
+        symv_tile<NRHS, WS, NT, int8_t>(reinterpret_cast<const int8_t*>(base + TILE_QHDR), tile,
+                                        diag, xr, rowacc, xcb, npad, cg, ro, wave, par, sh, colbuf,
+                                        sc);
       } else if (!diag && ro == 0 && wave == 0) {
 #pragma unroll
         for (int h = 0; h < NRHS; ++h) {
@@ -436,7 +479,9 @@ __global__ __launch_bounds__(64) void tile_maxabs_kernel(const double* __restric
   if (threadIdx.x == 0) tmax[tile] = m;
 }
 
-// Copy each tile into the compacted pool in its format.
+// Copy each tile into the compacted pool in its format (int16 / int8: the
+// tile's max |element| first, then the scale into the header and the
+// rounded integers after it).
 __global__ __launch_bounds__(256) void tile_pack_kernel(const double* __restrict__ M,
                                                         const uint8_t* __restrict__ fmt,
                                                         const int64_t* __restrict__ off,
@@ -446,11 +491,36 @@ __global__ __launch_bounds__(256) void tile_pack_kernel(const double* __restrict
   const uint8_t f = fmt[tile];
   if (f == TILE_ZERO) return;
   const double* tp = M + (tile << (2 * SM_LOGB));
+  char* dst = pool + off[tile];
+  if (f == TILE_F64 || f == TILE_F32) {
+    for (int i = threadIdx.x; i < SM_B * SM_B; i += 256) {
+      if (f == TILE_F64)
+        reinterpret_cast<double*>(dst)[i] = tp[i];
+      else
+        reinterpret_cast<float*>(dst)[i] = (float)tp[i];
+    }
+    return;
+  }
+  __shared__ double wmax[4];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < SM_B * SM_B; i += 256) m = fmax(m, fabs(tp[i]));
+  for (int w = 32; w > 0; w >>= 1) m = fmax(m, __shfl_xor(m, w));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+  const double qmax = f == TILE_I16 ? 32767.0 : 127.0;
+  // |x| * (qmax / m) <= qmax (1 + 2^-52): rint stays within the format's range
+  const double inv = m > 0.0 ? qmax / m : 0.0;
+  if (threadIdx.x == 0) {
+    reinterpret_cast<double*>(dst)[0] = m / qmax;
+    reinterpret_cast<double*>(dst)[1] = 0.0;
+  }
   for (int i = threadIdx.x; i < SM_B * SM_B; i += 256) {
-    if (f == TILE_F64)
-      reinterpret_cast<double*>(pool + off[tile])[i] = tp[i];
+    const int q = (int)rint(tp[i] * inv);
+    if (f == TILE_I16)
+      reinterpret_cast<int16_t*>(dst + TILE_QHDR)[i] = (int16_t)q;
     else
-      reinterpret_cast<float*>(pool + off[tile])[i] = (float)tp[i];
+      reinterpret_cast<int8_t*>(dst + TILE_QHDR)[i] = (int8_t)q;
   }
 }
 

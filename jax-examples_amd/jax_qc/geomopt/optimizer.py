@@ -36,11 +36,13 @@ class OptResult(NamedTuple):
 
 
 def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12, tau32=None,
-                        tile_f32=None, grad_tau=1e-13, grad_tau_density=0.0, grad_tau32=None,
+                        tile_f32=None, tile_i16=0.0, tile_i8=0.0, grad_tau=1e-13,
+                        grad_tau_density=0.0, grad_tau32=None,
                         prim_eps=None, fock_tau=None, fock_full_every=0, timings=None,
                         keep_store=False, grad_prim_eps=None):
     """Energy and gradient at ``coords``.  Mixed-path options: tau32 (FP32
-    ERI sums below that Schwarz bound), tile_f32 (FP32 supermatrix tiles),
+    ERI sums below that Schwarz bound), tile_f32 (FP32 supermatrix tiles;
+    with it, tile_i16 / tile_i8: int16 / int8 tiles with a per-tile scale),
     prim_eps (primitive-pair screening of the ERIs and derivative ERIs;
     grad_prim_eps, when set, screens the derivative ERIs' primitive pairs at
     its own level),
@@ -68,7 +70,7 @@ def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12,
             ints.ERI.ensure_P()
             ints.ERI.release_J()
         if tile_f32 is not None:
-            ints.ERI.use_compact(tau_f32=tile_f32)
+            ints.ERI.use_compact(tau_f32=tile_f32, tau_i16=tile_i16, tau_i8=tile_i8)
         if fock_tau is not None:
             ints.ERI.set_incremental(fock_tau, fock_full_every)
     t = mark("integrals", t)

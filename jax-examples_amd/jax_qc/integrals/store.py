@@ -30,14 +30,20 @@ from jax_qc import _native as nat
 from jax_qc.integrals import engine as eng
 
 
-TILE_ZERO, TILE_F32, TILE_F64 = 0, 1, 2
+TILE_ZERO, TILE_F32, TILE_F64, TILE_I16, TILE_I8 = 0, 1, 2, 3, 4
+# pool bytes per tile by format: 64 x 64 elements; the integer formats carry
+# a 16-byte header (the tile's FP64 scale, then padding)
+TILE_BYTES = {TILE_ZERO: 0, TILE_F32: 4 * 4096, TILE_F64: 8 * 4096,
+              TILE_I16: 16 + 2 * 4096, TILE_I8: 16 + 4096}
 
 
 @dataclass
 class CompactSupermatrix:
-    """A supermatrix re-packed tile by tile: tiles whose max |element| is
-    below ``tau_zero`` are dropped, below ``tau_f32`` stored in FP32, the rest
-    in FP64.  The Fock build accumulates in FP64 either way."""
+    """A supermatrix re-packed tile by tile: tiles whose max |element| m is
+    below ``tau_zero`` are dropped; below ``tau_i8`` / ``tau_i16`` they are
+    held as int8 / int16 multiples of a per-tile scale m/127 or m/32767 (each
+    element off by at most m/254 or m/65534); below ``tau_f32`` in FP32; the
+    rest in FP64.  The Fock build accumulates in FP64 either way."""
 
     pool: torch.Tensor      # uint8 byte pool
     fmt: torch.Tensor       # uint8 per tile
@@ -49,6 +55,23 @@ class CompactSupermatrix:
     bytes: int
     tau_f32: float
     tau_zero: float
+    n_i16: int = 0
+    n_i8: int = 0
+    tau_i16: float = 0.0
+    tau_i8: float = 0.0
+
+
+def tile_formats(tm: np.ndarray, tau_f32: float, tau_zero: float = 1e-14,
+                 tau_i16: float = 0.0, tau_i8: float = 0.0) -> np.ndarray:
+    """Per-tile format codes from the tiles' max |element| tm: zero below
+    tau_zero, int8 below tau_i8, int16 below tau_i16, FP32 below tau_f32,
+    FP64 otherwise."""
+    fmt = np.full(tm.shape, TILE_F64, dtype=np.uint8)
+    fmt[tm < tau_f32] = TILE_F32
+    fmt[tm < tau_i16] = TILE_I16
+    fmt[tm < tau_i8] = TILE_I8
+    fmt[tm < tau_zero] = TILE_ZERO
+    return fmt
 
 
 class ERIStore:
@@ -76,19 +99,21 @@ class ERIStore:
         self._work, self._x, self._y, self._xm = {}, {}, {}, {}
         self.inc = None
 
-    def use_compact(self, tau_f32: float = 1e-4, tau_zero: float = 1e-14):
+    def use_compact(self, tau_f32: float = 1e-4, tau_zero: float = 1e-14,
+                    tau_i16: float = 0.0, tau_i8: float = 0.0):
         """Replace the assembled supermatrices (P for RHF; J and K for UHF)
         by their compacted mixed forms.  With neither P nor K assembled yet,
         P is assembled first (the RHF case); a store that already holds K is
         taken as UHF and no P is built."""
+        kw = dict(tau_f32=tau_f32, tau_zero=tau_zero, tau_i16=tau_i16, tau_i8=tau_i8)
         if self.P is None and self.K is None:
             self.ensure_P()
         if isinstance(self.P, torch.Tensor):
-            self.P = self.compact(self.P, tau_f32, tau_zero)
+            self.P = self.compact(self.P, **kw)
         if isinstance(self.K, torch.Tensor):
-            self.K = self.compact(self.K, tau_f32, tau_zero)
+            self.K = self.compact(self.K, **kw)
         if isinstance(self.J, torch.Tensor) and self.K is not None:
-            self.J = self.compact(self.J, tau_f32, tau_zero)
+            self.J = self.compact(self.J, **kw)
         return self
 
     def _assemble_from_J(self, what):
@@ -119,17 +144,22 @@ class ERIStore:
         self.ensure_P()
         self.J = None
 
-    def compact(self, M: torch.Tensor, tau_f32: float = 1e-4, tau_zero: float = 1e-14
-                ) -> CompactSupermatrix:
-        """Mixed-precision, zero-skipping copy of supermatrix M."""
+    def compact(self, M: torch.Tensor, tau_f32: float = 1e-4, tau_zero: float = 1e-14,
+                tau_i16: float = 0.0, tau_i8: float = 0.0) -> CompactSupermatrix:
+        """Mixed-precision, zero-skipping copy of supermatrix M (formats by
+        tile max |element|: see CompactSupermatrix; tau_i16 = tau_i8 = 0 keeps
+        the integer formats off)."""
+        if tau_i8 > max(tau_i16, tau_f32) or tau_i16 > tau_f32:
+            raise ValueError("tile thresholds must satisfy tau_i8 <= tau_i16 <= tau_f32")
         T = self.T
         nt = T * (T + 1) // 2
         tmax = torch.empty(nt, dtype=torch.float64, device=self.dev)
         nat.call("qc_sm_tile_maxabs", nat.ptr(M), self.nbf, nat.ptr(tmax), nat.stream_ptr())
         tm = tmax.cpu().numpy()
-        fmt = np.where(tm < tau_zero, TILE_ZERO, np.where(tm < tau_f32, TILE_F32, TILE_F64))
-        fmt = fmt.astype(np.uint8)
-        size = np.where(fmt == TILE_F64, 8 * 4096, np.where(fmt == TILE_F32, 4 * 4096, 0))
+        fmt = tile_formats(tm, tau_f32, tau_zero, tau_i16, tau_i8)
+        size = np.zeros(nt, dtype=np.int64)
+        for f, b in TILE_BYTES.items():
+            size[fmt == f] = b
         off = np.zeros(nt, dtype=np.int64)
         off[1:] = np.cumsum(size)[:-1]
         total = int(size.sum())
@@ -140,8 +170,10 @@ class ERIStore:
                  nat.stream_ptr())
         # FP32 maxima rounded up, so a screen never undershoots a tile's bound
         t32 = torch.from_numpy(np.nextafter(tm.astype(np.float32), np.float32(np.inf))).to(self.dev)
-        return CompactSupermatrix(pool, fd, od, t32, int((fmt == 0).sum()), int((fmt == 1).sum()),
-                                  int((fmt == 2).sum()), total, tau_f32, tau_zero)
+        cnt = {f: int((fmt == f).sum()) for f in TILE_BYTES}
+        return CompactSupermatrix(pool, fd, od, t32, cnt[TILE_ZERO], cnt[TILE_F32], cnt[TILE_F64],
+                                  total, tau_f32, tau_zero, cnt[TILE_I16], cnt[TILE_I8],
+                                  tau_i16, tau_i8)
 
     # -- Fock builds --------------------------------------------------------
     def bytes_per_matrix(self) -> int:

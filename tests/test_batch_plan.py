@@ -47,3 +47,20 @@ def test_batched_plan_is_union_of_single_plans(tau, tau32):
     if tau == 0.0:  # unscreened: every canonical quartet of every molecule, once
         n = sum(len(t.sp_class) * (len(t.sp_class) + 1) // 2 for t in tables)
         assert len(got) == n
+
+
+def test_tile_formats_follow_the_thresholds():
+    """Format codes by tile max |element|: zero < tau_zero <= int8 < tau_i8
+    <= int16 < tau_i16 <= FP32 < tau_f32 <= FP64; thresholds of 0 switch the
+    integer formats off."""
+    import numpy as np
+
+    from jax_qc.integrals.store import (TILE_F32, TILE_F64, TILE_I16, TILE_I8, TILE_ZERO,
+                                        tile_formats)
+
+    tm = np.array([0.0, 1e-15, 1e-14, 5e-8, 1e-7, 2e-5, 3e-5, 5e-3, 1e-2, 1.0])
+    f = tile_formats(tm, tau_f32=1e-2, tau_zero=1e-14, tau_i16=3e-5, tau_i8=1e-7)
+    assert list(f) == [TILE_ZERO, TILE_ZERO, TILE_I8, TILE_I8, TILE_I16, TILE_I16, TILE_F32,
+                       TILE_F32, TILE_F64, TILE_F64]
+    f0 = tile_formats(tm, tau_f32=1e-2, tau_zero=1e-14)
+    assert list(f0) == [TILE_ZERO, TILE_ZERO] + [TILE_F32] * 6 + [TILE_F64] * 2

@@ -54,6 +54,53 @@ def test_compact_fp32_error_bounded(dimer):
     assert (a - b).abs().max() <= bound
 
 
+@pytest.mark.parametrize("fmt", ["i16", "i8", "all"])
+def test_compact_integer_tiles_error_bounded(dimer, fmt):
+    """int16 / int8 tiles: every element is off by at most max|tile| / 65534
+    (int16) or / 254 (int8), so y = M x is off by at most the contraction of
+    |x| with a supermatrix of those per-tile bounds."""
+    from jax_qc.integrals.store import ERIStore, TILE_F32, TILE_F64, TILE_I16, TILE_I8
+
+    mol, bs = dimer
+    st = ERIStore(bs, tau=1e-12)
+    P = st.ensure_P()
+    if fmt == "all":
+        # all four formats: thresholds at the quartiles of the tile maxima
+        q = np.quantile(st.compact(P, tau_f32=0.0, tau_zero=0.0).tmax.cpu().numpy(),
+                        [0.25, 0.5, 0.75])
+        kw = dict(tau_i8=q[0], tau_i16=q[1], tau_f32=q[2])
+    else:
+        kw = {"i16": dict(tau_f32=np.inf, tau_i16=np.inf),
+              "i8": dict(tau_f32=np.inf, tau_i16=np.inf, tau_i8=np.inf)}[fmt]
+    C = st.compact(P, tau_zero=0.0, **kw)
+    f = C.fmt.cpu().numpy()
+    if fmt == "i16":
+        assert C.n_i16 == len(f) and C.n_i8 == 0
+    elif fmt == "i8":
+        assert C.n_i8 == len(f)
+    else:
+        assert C.n_i16 > 0 and C.n_i8 > 0 and C.n_f32 > 0 and C.n_f64 > 0
+    n = bs.n_basis
+    g = torch.Generator(device="cpu").manual_seed(3)
+    D = torch.randn(n, n, dtype=torch.float64, generator=g).cuda()
+    D = D + D.T
+    a = st.contract(P, D)
+    b = st.contract(C, D)
+    tm = C.tmax.double()
+    fd = C.fmt.long()
+    per = torch.zeros_like(tm)
+    per[fd == TILE_I16] = tm[fd == TILE_I16] / 65534.0
+    per[fd == TILE_I8] = tm[fd == TILE_I8] / 254.0
+    per[fd == TILE_F32] = tm[fd == TILE_F32] * 6e-8
+    per[fd == TILE_F64] = tm[fd == TILE_F64] * 1e-14  # summation order only
+    B = per.repeat_interleave(64 * 64)
+    assert B.numel() == P.numel()
+    bound = st.contract(B, D.abs()).abs() * (1 + 1e-6) + 1e-15
+    err = (a - b).abs()
+    assert bool((err <= bound).all()), float((err - bound).max())
+    assert float(err.max()) > 0.0  # the quantisation is in effect
+
+
 def test_mixed_scf_energy_within_1uEh(dimer):
     import jax_qc
 
