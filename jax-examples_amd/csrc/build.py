@@ -47,6 +47,13 @@ GRAD_PER_THREAD_F32 = GRAD_PER_THREAD | {"2200"}
 # FP32 (dd|ps) spill 100-700 registers)
 GRAD_PRIM = {"1111", "2011"}
 GRAD_PRIM_F32 = GRAD_PRIM | {"2110", "2111", "2120"}
+# ket Cartesian pairs per launch of that kernel, (FP64, FP32), 1 where not
+# listed.  Measured on the (H2O)_16 gradient (1 -> chosen): FP32 (ds|pp)
+# 142.9 -> 21.8 ms at 9 (53.9 at 3), FP32 (pp|pp) 102.9 -> 39.4 ms at 3
+# (47.3 at 9); larger chunks lost for FP64 (pp|pp) (444 -> 1144 ms at 3),
+# FP64 (ds|pp) (694 -> 876 at 9) and FP32 (dp|ps), (dp|pp), (dp|ds)
+# (105 -> 354, 92 -> 423, 41 -> 162 ms at 3)
+GRAD_PRIM_CHUNK = {"2011": (1, 9), "1111": (1, 3)}
 STATIC_SOURCES = ["capi.hip", "oneint.hip", "jk.hip", "grad.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -82,6 +89,7 @@ def _generate():
             # rough cost for scheduling: total angular momentum, fp64 first
             out.append((sum(c) * 2 + (prec == "f64"), path))
         path = os.path.join(GEN, f"grad2e_{tag}.hip")
+        ch = GRAD_PRIM_CHUNK.get(tag, (1, 1))
         text = ('#include "grad_prim.h"\n'
                 "namespace qcmd {\n"
                 f"hipError_t grad2e_launchw_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
@@ -99,11 +107,11 @@ def _generate():
                    if tag in GRAD_PER_THREAD_F32 else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
                 + "}\n"
                 f"hipError_t grad2e_launchp_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                + (f"  return grad2e_launch_prim<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
+                + (f"  return grad2e_launch_prim<{c[0]}, {c[1]}, {c[2]}, {c[3]}, double, {ch[0]}>(a, s);\n"
                    if tag in GRAD_PRIM else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
                 + "}\n"
                 f"hipError_t grad2e_launchp32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                + (f"  return grad2e_launch_prim<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float>(a, s);\n"
+                + (f"  return grad2e_launch_prim<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float, {ch[1]}>(a, s);\n"
                    if tag in GRAD_PRIM_F32 else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
                 + "}\n}  // namespace qcmd\n")
         if not os.path.exists(path) or open(path).read() != text:

@@ -268,10 +268,11 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   const bool has_thread32 = has_thread || (cab == 5 && ccd == 0);
   // one lane per primitive quartet where that form measured faster on
   // (H2O)_16 (wave form -> per-primitive form): FP64 (pp|pp) 773 -> 453 ms,
-  // (ds|pp) 1183 -> 711 ms; FP32 (pp|pp) 121 -> 104, (dp|ps) 123 -> 106,
-  // (dp|pp) 111 -> 94, (dp|ds) 56 -> 41 ms (FP32 (ds|pp) 142 vs 143: wave)
+  // (ds|pp) 1183 -> 711 ms; FP32 (pp|pp) 121 -> 39 (three ket pairs per
+  // launch), (ds|pp) 142 -> 22 (all nine per launch), (dp|ps) 123 -> 106,
+  // (dp|pp) 111 -> 94, (dp|ds) 56 -> 41 ms
   const bool prim64 = (cab == 2 && ccd == 2) || (cab == 3 && ccd == 2);
-  const bool prim32 = (cab == 2 && ccd == 2) || (cab == 4 && ccd >= 1 && ccd <= 3);
+  const bool prim32 = (cab == 2 && ccd == 2) || (cab == 3 && ccd == 2) || (cab == 4 && ccd >= 1 && ccd <= 3);
   const int var = in->variant != 0
                       ? in->variant
                       : ((in->fp32 ? has_thread32 : has_thread) ? 1 : ((in->fp32 ? prim32 : prim64) ? 3 : 2));

@@ -362,21 +362,25 @@ __global__ __launch_bounds__(64) void grad2e_prim_kernel(Grad2eArgs args) {
   }
 }
 
-// one launch per ket Cartesian pair cd (each adds its share of the
-// gradient): a lane then holds R and the h lists of one cd only
-template <int LA, int LB, int LC, int LD, typename Real, int CD0, int NCD>
+// CH ket Cartesian pairs cd per launch, each launch adding its share of the
+// gradient (a lane holds R and the h lists of one cd at a time either way).
+// Every launch repeats the quartet's G block, E tables, Boys values and R,
+// so fewer launches pay where the larger unrolled body still fits the
+// registers; the build picks CH per class and precision by measurement.
+template <int LA, int LB, int LC, int LD, typename Real, int CD0, int NCD, int CH>
 void grad2e_launch_prim_chunks(dim3 grid, const Grad2eArgs& a, hipStream_t s) {
   if constexpr (CD0 < NCD) {
-    hipLaunchKernelGGL((grad2e_prim_kernel<LA, LB, LC, LD, Real, CD0, CD0 + 1>), grid, dim3(64), 0, s, a);
-    grad2e_launch_prim_chunks<LA, LB, LC, LD, Real, CD0 + 1, NCD>(grid, a, s);
+    constexpr int CD1 = CD0 + CH < NCD ? CD0 + CH : NCD;
+    hipLaunchKernelGGL((grad2e_prim_kernel<LA, LB, LC, LD, Real, CD0, CD1>), grid, dim3(64), 0, s, a);
+    grad2e_launch_prim_chunks<LA, LB, LC, LD, Real, CD1, NCD, CH>(grid, a, s);
   }
 }
 
-template <int LA, int LB, int LC, int LD, typename Real = double>
+template <int LA, int LB, int LC, int LD, typename Real = double, int CH = 1>
 hipError_t grad2e_launch_prim(const Grad2eArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
-  grad2e_launch_prim_chunks<LA, LB, LC, LD, Real, 0, ncart(LC) * ncart(LD)>(grid_1d(a.nquartets, 64),
-                                                                              a, s);
+  grad2e_launch_prim_chunks<LA, LB, LC, LD, Real, 0, ncart(LC) * ncart(LD), CH>(
+      grid_1d(a.nquartets, 64), a, s);
   return hipGetLastError();
 }
 
