@@ -54,7 +54,7 @@ GRAD_PRIM_F32 = GRAD_PRIM | {"2110", "2111", "2120"}
 # FP64 (ds|pp) (694 -> 876 at 9) and FP32 (dp|ps), (dp|pp), (dp|ds)
 # (105 -> 354, 92 -> 423, 41 -> 162 ms at 3)
 GRAD_PRIM_CHUNK = {"2011": (1, 9), "1111": (1, 3)}
-STATIC_SOURCES = ["capi.hip", "oneint.hip", "jk.hip", "grad.hip"]
+STATIC_SOURCES = ["capi.hip", "oneint.hip", "jk.hip", "grad.hip", "purify.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

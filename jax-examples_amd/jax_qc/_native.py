@@ -81,6 +81,9 @@ _SIGS = {
                   c_int, c_int, c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p,
                   c_void_p],
     "qc_tiles_maxabs": [c_void_p, c_i64, c_void_p, c_void_p],
+    "qc_tc2_partials": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
+    "qc_tc2_update": [c_void_p, c_void_p, c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p,
+                      c_void_p],
     "qc_one_electron_batch": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                               c_int, c_void_p, c_void_p, c_void_p, c_i64, c_void_p, c_void_p,
                               c_void_p],
@@ -120,6 +123,8 @@ def lib():
             L.qc_last_error.argtypes = []
             L.qc_symv_workspace_bytes.restype = c_i64
             L.qc_symv_workspace_bytes.argtypes = [c_i64, c_int]
+            L.qc_tc2_partials_len.restype = c_i64
+            L.qc_tc2_partials_len.argtypes = [c_int]
             L.qc_batch_items.restype = c_i64
             L.qc_batch_items.argtypes = [ctypes.POINTER(c_i64), c_int, c_int,
                                          ctypes.POINTER(c_i64)]
@@ -132,7 +137,8 @@ def lib():
 
 
 def exported_symbols():
-    return (list(_SIGS) + ["qc_last_error", "qc_symv_workspace_bytes", "qc_batch_items"]
+    return (list(_SIGS) + ["qc_last_error", "qc_symv_workspace_bytes", "qc_batch_items",
+                           "qc_tc2_partials_len"]
             + list(_OPTIONAL_SIGS))
 
 

@@ -23,9 +23,11 @@ import torch
 # off: every SCF step diagonalises, as the reference does
 ENABLED = True
 # On the GPU the iteration replays two HIP graphs (the start X0 and a chunk of
-# check_every steps) instead of launching ~6 small kernels per step from
-# Python: the launches, not the 2 x n^3 products, set the cost at a few
-# hundred functions.  Off: every step is launched eagerly.
+# check_every steps, each step three launches: qc_tc2_partials, the batched
+# product, qc_tc2_update, with the branch taken on the device) instead of
+# launching ~8 small kernels per step from Python: the launches, not the
+# 2 x n^3 products, set the cost at a few hundred functions.  Off: every step
+# is launched eagerly with torch operations.
 USE_GRAPHS = True
 _GRAPH_CACHE: "OrderedDict" = None
 _GRAPH_CACHE_MAX = 8
@@ -36,12 +38,21 @@ class _TC2Graphs:
     shape; inputs and outputs are the static tensors F, X, idem."""
 
     def __init__(self, B, n, n_occ, dev, dt, steps):
+        from jax_qc import _native as nat
+
+        self.nat = nat
         self.F = torch.zeros(B, n, n, dtype=dt, device=dev)
         self.X = torch.zeros_like(self.F)
+        self.X2 = torch.zeros_like(self.F)
+        self.part = torch.zeros(int(nat.lib().qc_tc2_partials_len(B)), dtype=dt, device=dev)
         self.idem = torch.zeros(B, dtype=dt, device=dev)
         self.nocc = torch.tensor([float(k) for k in n_occ], dtype=dt, device=dev)
         self.eye = torch.eye(n, dtype=dt, device=dev)
         self.steps = steps
+        self.B, self.n = B, n
+        # chunks the last call needed: the next call replays all but one of
+        # them before it first reads the convergence measure back
+        self.last_chunks = 1
         cur = torch.cuda.current_stream(dev)
         side = torch.cuda.Stream(dev)
         side.wait_stream(cur)
@@ -67,16 +78,12 @@ class _TC2Graphs:
         self.X.copy_((hi[:, None, None] * self.eye - F) / (hi - lo)[:, None, None])
 
     def _chunk(self):
-        X = self.X
-        for i in range(self.steps):
-            X2 = torch.bmm(X, X)
-            trX = torch.diagonal(X, dim1=1, dim2=2).sum(-1)
-            trX2 = torch.diagonal(X2, dim1=1, dim2=2).sum(-1)
-            Xn = torch.where((trX > self.nocc)[:, None, None], X2, 2.0 * X - X2)
-            if i == self.steps - 1:
-                self.idem.copy_((trX - trX2).abs())
-            X = Xn
-        self.X.copy_(X)
+        nat, s = self.nat, self.nat.stream_ptr()
+        for _ in range(self.steps):
+            nat.call("qc_tc2_partials", nat.ptr(self.X), self.n, self.B, nat.ptr(self.part), s)
+            torch.bmm(self.X, self.X, out=self.X2)
+            nat.call("qc_tc2_update", nat.ptr(self.X), nat.ptr(self.X2), nat.ptr(self.part),
+                     nat.ptr(self.nocc), self.n, self.B, nat.ptr(self.idem), None, s)
 
 
 def _graphs_for(B, n, n_occ, dev, dt, steps):
@@ -115,16 +122,21 @@ def occupied_projectors(F: torch.Tensor, n_occ: Sequence[int], tol: float = 1e-1
             elif k > 0:
                 return None
         return out
-    if USE_GRAPHS and F.is_cuda and max_iter % check_every == 0:
+    if USE_GRAPHS and F.is_cuda and max_iter % check_every == 0 and dt == torch.float64:
         g = _graphs_for(B, n, n_occ, dev, dt, check_every)
         g.F.copy_(F)
         g.g_start.replay()
-        for _ in range(max_iter // check_every):
+        nchunk = max_iter // check_every
+        blind = min(max(g.last_chunks - 1, 0), nchunk)
+        for c in range(nchunk):
             g.g_chunk.replay()
-            if float(g.idem.max()) < tol:
+            # idem measures the matrix before the chunk's last step
+            if c + 1 > blind and float(g.idem.max()) < tol:
+                g.last_chunks = c + 1
                 X = 0.5 * (g.X + g.X.transpose(1, 2))
                 trn = torch.diagonal(X, dim1=1, dim2=2).sum(-1)
                 return X if float((trn - nocc).abs().max()) < 1e-6 else None
+        g.last_chunks = 1
         return None
     d = torch.diagonal(F, dim1=1, dim2=2)
     r = F.abs().sum(dim=2) - d.abs()
