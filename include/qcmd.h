@@ -259,8 +259,11 @@ typedef struct {
                           classes only), 2 one wave per quartet with the bra
                           side contracted with G first, 3 one wave per
                           quartet with one lane per primitive quartet
-                          (classes with la+lb+lc+ld <= 6 except those with
-                          a thread kernel) */
+                          ((pp|pp), (ds|pp); in FP32 also (dp|ps), (dp|pp),
+                          (dp|ds)), 4 form 2 with that contraction on the
+                          matrix cores ((pp|pp), (ds|pp), (dp|ps), (dp|pp),
+                          (dp|dp)); a class without the form asked for
+                          returns an error */
   const double* schwarz; /* optional (with tau_density > 0): per shell pair Q */
   double tau_density;    /* skip a quartet when max|G_abcd| Q_ab Q_cd < tau_density */
   const float* pairs32;  /* fp32 = 1: FP32 primitive-pair table */

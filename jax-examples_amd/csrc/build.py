@@ -54,6 +54,9 @@ GRAD_PRIM_F32 = GRAD_PRIM | {"2110", "2111", "2120"}
 # FP64 (ds|pp) (694 -> 876 at 9) and FP32 (dp|ps), (dp|pp), (dp|ds)
 # (105 -> 354, 92 -> 423, 41 -> 162 ms at 3)
 GRAD_PRIM_CHUNK = {"2011": (1, 9), "1111": (1, 3)}
+# derivative-ERI classes that also get the wave kernel with its Y phase on the
+# matrix cores (variant 4; measured against the VALU Y phase)
+GRAD_WAVE_MFMA = {"1111", "2011", "2110", "2111", "2121"}
 STATIC_SOURCES = ["capi.hip", "oneint.hip", "jk.hip", "grad.hip", "purify.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -98,6 +101,14 @@ def _generate():
                 f"hipError_t grad2e_launchw32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float>(a, s);\n"
                 "}\n"
+                f"hipError_t grad2e_launchwm_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                + (f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}, double, true>(a, s);\n"
+                   if tag in GRAD_WAVE_MFMA else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
+                + "}\n"
+                f"hipError_t grad2e_launchwm32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                + (f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float, true>(a, s);\n"
+                   if tag in GRAD_WAVE_MFMA else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
+                + "}\n"
                 f"hipError_t grad2e_launcht_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
                 + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
                    if tag in GRAD_PER_THREAD else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")

@@ -184,7 +184,9 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
   hipError_t grad2e_launchw_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);   \
   hipError_t grad2e_launchw32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);  \
   hipError_t grad2e_launchp_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);    \
-  hipError_t grad2e_launchp32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
+  hipError_t grad2e_launchp32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);  \
+  hipError_t grad2e_launchwm_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);   \
+  hipError_t grad2e_launchwm32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
 QC_ERI_CLASSES(QC_GDECL)
 #undef QC_GDECL
 
@@ -258,7 +260,7 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   if (a.tau_density > 0.0 && !a.schwarz) return set_error("qc_grad2e: tau_density without schwarz");
   a.boys64 = boys_table64();
   if (!a.boys64) return set_error("qc_grad2e: qc_init not called on the current device");
-  if (in->variant < 0 || in->variant > 3) return set_error("qc_grad2e: variant %d", in->variant);
+  if (in->variant < 0 || in->variant > 4) return set_error("qc_grad2e: variant %d", in->variant);
   // default (measured on (H2O)_8 and (H2O)_16): one thread per quartet for
   // every class that has that kernel (ket (ss) up to (dp|ss), ket (ps) up to
   // (ds|ps), and (ds|ds)), one wave with the bra side contracted first for
@@ -282,12 +284,15 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   if (cab == (A) * ((A) + 1) / 2 + (B) && ccd == (C) * ((C) + 1) / 2 + (D))                    \
     e = var == 1 ? (in->fp32 ? grad2e_launcht32_##A##B##C##D(a, s) : grad2e_launcht_##A##B##C##D(a, s))  \
       : var == 3 ? (in->fp32 ? grad2e_launchp32_##A##B##C##D(a, s) : grad2e_launchp_##A##B##C##D(a, s))  \
+      : var == 4 ? (in->fp32 ? grad2e_launchwm32_##A##B##C##D(a, s) : grad2e_launchwm_##A##B##C##D(a, s)) \
                  : (in->fp32 ? grad2e_launchw32_##A##B##C##D(a, s) : grad2e_launchw_##A##B##C##D(a, s));
   QC_ERI_CLASSES(QC_GC)
 #undef QC_GC
   if (e == hipErrorNotSupported)
     return set_error("qc_grad2e: class (%d, %d) has no %s kernel", cab, ccd,
-                     var == 3 ? "one-lane-per-primitive-quartet" : "one-thread-per-quartet");
+                     var == 3   ? "one-lane-per-primitive-quartet"
+                     : var == 4 ? "matrix-core Y-phase wave"
+                                : "one-thread-per-quartet");
   return check(e, "qc_grad2e launch");
 }
 

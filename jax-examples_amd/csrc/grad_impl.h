@@ -495,7 +495,41 @@ struct GradWave {
     Real f[KE][L1 + 1];
     Real misc[KE][4];           // X, Y, Z, pref
     double wmax[NT / 64];  // per-wave max |G| (density screening)
+    // MFMA form of the Y phase: per bra Cartesian pair ab, the Cartesian
+    // exponents a_x, a_y, a_z in bits 10-11, 12-13, 14-15 and b_x, b_y, b_z
+    // in bits 16-17, 18-19, 20-21
+    int abinfo[NAB];
   };
+};
+
+// The Y phase as matrix products on the matrix cores (FP64: v_mfma_f64_
+// 16x16x4f64; FP32: v_mfma_f32_16x16x4f32):
+//   Y_o[k][cl] = sum_ab A_o[k][ab] G[ab][cd0 + cl],  o = 0 (E^{ab}_k),
+//   1..3 (dE^{ab}_k / dA_x,y,z)
+// with the Hermite index k as the row (16 per tile), the chunk's ket pair cl
+// as the column (16 per tile) and ab as the reduction (4 per step).  Lane l
+// supplies A[16 kt + l % 16][4 s + l / 16] (all four o from one set of E
+// reads) and B[4 s + l / 16][16 ct + l % 16], and holds the four results
+// D[16 kt + row(l / 16, r)][16 ct + l % 16], r < 4, with row(g, r) = 4 g + r
+// for the FP32 instruction and g + 4 r for the FP64 one (both read off the
+// device with one-hot inputs).
+template <typename Real>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
+};
+template <>
+struct MfmaAcc<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
 // Waves per SIMD that the LDS use of a class allows (4 SIMDs and 160 KB of
@@ -510,7 +544,7 @@ constexpr int grad_wave_occupancy() {
   return w < 1 ? 1 : (w > cap ? cap : w);
 }
 
-template <int LA, int LB, int LC, int LD, typename Real>
+template <int LA, int LB, int LC, int LD, typename Real, bool YM = false>
 __global__ __launch_bounds__((GradWave<LA, LB, LC, LD, Real>::NT)) __attribute__((amdgpu_waves_per_eu(grad_wave_occupancy<LA, LB, LC, LD, Real>())))
 void grad2e_wave_kernel(Grad2eArgs args) {
   using W = GradWave<LA, LB, LC, LD, Real>;
@@ -581,6 +615,15 @@ void grad2e_wave_kernel(Grad2eArgs args) {
               iv = v > 0 ? herm_index(t, u, v - 1) : none;
     sh.hk[k] = ((t * RB + u) * RB + v) | (it << 10) | (iu << 17) | (iv << 24);
   }
+  if constexpr (YM) {
+    for (int ab = tid; ab < NAB; ab += NT) {
+      const int a = ab / NB, b = ab - a * NB;
+      const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
+      const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
+      // the three directions' tables share one layout: (i (LB+1) + j) (LAB+2)
+      sh.abinfo[ab] = (ax << 10) | (ay << 12) | (az << 14) | (bx << 16) | (by << 18) | (bz << 20);
+    }
+  }
   // n / d for 0 <= n < 2^11, 1 <= d <= 2^9 by one multiply: inv = ceil(2^20 / d)
   // (exact over that range; n = ket primitive x ket pair of a chunk, d = ket pairs)
   static_assert(KC * NCDC < 2048 && NCDC <= 512, "div_small range");
@@ -607,6 +650,69 @@ void grad2e_wave_kernel(Grad2eArgs args) {
     for (int cd0 = 0; cd0 < NCD; cd0 += NCDC) {
       const int ncdc = min(NCDC, NCD - cd0);
       __syncthreads();
+      if constexpr (YM) {
+        using MA = MfmaAcc<Real>;
+        typedef typename MA::v4 v4;
+        constexpr int KT = (NHB1 + 15) / 16, ES = (LAB + 2), JS = (LB + 1) * (LAB + 2);
+        const int ct_n = (ncdc + 15) / 16;
+        const int q = lane >> 4, c16 = lane & 15;
+        for (int tile = tid >> 6; tile < KT * ct_n; tile += NT / 64) {
+          const int kt = tile % KT, ct = tile / KT;
+          const int krow = kt * 16 + c16;  // this lane's A row
+          const int ccol = ct * 16 + c16;  // this lane's B column
+          const bool kin = krow < NHB1, cin = ccol < ncdc;
+          int t = 0, u = 0, v = 0;
+          if (kin) {
+            const int off = sh.hk[krow] & 1023;
+            t = off / (RB * RB);
+            u = (off / RB) % RB;
+            v = off % RB;
+          }
+          v4 y0 = {0, 0, 0, 0}, yx = y0, yy = y0, yz = y0;
+#pragma unroll
+          for (int ab0 = 0; ab0 < NAB; ab0 += 4) {
+            const int ab = ab0 + q;
+            Real a0 = (Real)0, ax_ = (Real)0, ay_ = (Real)0, az_ = (Real)0, g = (Real)0;
+            if (ab < NAB) {
+              const int inf = sh.abinfo[ab];
+              const int ax = (inf >> 10) & 3, ay = (inf >> 12) & 3, az = (inf >> 14) & 3;
+              const int bx = (inf >> 16) & 3, by = (inf >> 18) & 3, bz = (inf >> 20) & 3;
+              if (cin) g = sh.gam[ab * NCD + cd0 + ccol];
+              if (kin) {
+                const Real* ex = sh.eb[0] + bx * ES + t;
+                const Real* ey = sh.eb[1] + by * ES + u;
+                const Real* ez = sh.eb[2] + bz * ES + v;
+                const Real e0x = ex[ax * JS], e0y = ey[ay * JS], e0z = ez[az * JS];
+                Real dx = al2 * ex[(ax + 1) * JS], dy = al2 * ey[(ay + 1) * JS], dz = al2 * ez[(az + 1) * JS];
+                if (ax > 0) dx -= (Real)ax * ex[(ax - 1) * JS];
+                if (ay > 0) dy -= (Real)ay * ey[(ay - 1) * JS];
+                if (az > 0) dz -= (Real)az * ez[(az - 1) * JS];
+                const Real eyz = e0y * e0z;
+                a0 = e0x * eyz;
+                ax_ = dx * eyz;
+                ay_ = e0x * dy * e0z;
+                az_ = e0x * e0y * dz;
+              }
+            }
+            y0 = MA::step(a0, g, y0);
+            yx = MA::step(ax_, g, yx);
+            yy = MA::step(ay_, g, yy);
+            yz = MA::step(az_, g, yz);
+          }
+          if (cin) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int k = kt * 16 + MA::row(q, r);
+              if (k < NHB1) {
+                sh.y0[ccol][k] = y0[r];
+                sh.ya[0][ccol][k] = yx[r];
+                sh.ya[1][ccol][k] = yy[r];
+                sh.ya[2][ccol][k] = yz[r];
+              }
+            }
+          }
+        }
+      } else
       // Y for this bra primitive and ket chunk: per item (cd, tuv), the
       // E columns at (t, u, v) in registers, then every ab unrolled
       for (int it = tid; it < ncdc * NHB1; it += NT) {
@@ -798,11 +904,11 @@ void grad2e_wave_kernel(Grad2eArgs args) {
   }
 }
 
-template <int LA, int LB, int LC, int LD, typename Real = double>
+template <int LA, int LB, int LC, int LD, typename Real = double, bool YM = false>
 hipError_t grad2e_launch_wave(const Grad2eArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   constexpr int NT = GradWave<LA, LB, LC, LD, Real>::NT;
-  hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD, Real>), grid_1d(a.nquartets, NT), dim3(NT), 0, s, a);
+  hipLaunchKernelGGL((grad2e_wave_kernel<LA, LB, LC, LD, Real, YM>), grid_1d(a.nquartets, NT), dim3(NT), 0, s, a);
   return hipGetLastError();
 }
 
