@@ -260,7 +260,8 @@ typedef struct {
                           side contracted with G first, 3 one wave per
                           quartet with one lane per primitive quartet
                           ((pp|pp), (ds|pp); in FP32 also (dp|ps), (dp|pp),
-                          (dp|ds)), 4 form 2 with that contraction on the
+                          (dp|ds), (dp|dp), (dd|ps), (dd|pp)), 4 form 2
+                          with that contraction on the
                           matrix cores ((pp|pp), (ds|pp), (dp|ps), (dp|pp),
                           (dp|dp)); a class without the form asked for
                           returns an error */

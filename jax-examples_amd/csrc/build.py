@@ -46,7 +46,10 @@ GRAD_PER_THREAD_F32 = GRAD_PER_THREAD | {"2200"}
 # FP32 (ds|pp) 134, (dp|pp) 233, (dp|ds) 229; FP64 (dp|ps) and (dd|ss) and
 # FP32 (dd|ps) spill 100-700 registers)
 GRAD_PRIM = {"1111", "2011"}
-GRAD_PRIM_F32 = GRAD_PRIM | {"2110", "2111", "2120"}
+# FP32 also (dp|ps), (dp|pp), (dp|ds) and, measured on the (H2O)_16 mixed
+# gradient against the wave form, (dp|dp) 79.5 -> 40.8 ms, (dd|pp) 25.7 ->
+# 18.3, (dd|ps) 22.0 -> 19.4 ((pp|ps) lost: 22.6 ms thread form, 246 ms)
+GRAD_PRIM_F32 = GRAD_PRIM | {"2110", "2111", "2120", "2121", "2211", "2210"}
 # ket Cartesian pairs per launch of that kernel, (FP64, FP32), 1 where not
 # listed.  Measured on the (H2O)_16 gradient (1 -> chosen): FP32 (ds|pp)
 # 142.9 -> 21.8 ms at 9 (53.9 at 3), FP32 (pp|pp) 102.9 -> 39.4 ms at 3
@@ -91,44 +94,40 @@ def _generate():
                     fh.write(text)
             # rough cost for scheduling: total angular momentum, fp64 first
             out.append((sum(c) * 2 + (prec == "f64"), path))
-        path = os.path.join(GEN, f"grad2e_{tag}.hip")
         ch = GRAD_PRIM_CHUNK.get(tag, (1, 1))
-        text = ('#include "grad_prim.h"\n'
-                "namespace qcmd {\n"
-                f"hipError_t grad2e_launchw_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
-                "}\n"
-                f"hipError_t grad2e_launchw32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float>(a, s);\n"
-                "}\n"
-                f"hipError_t grad2e_launchwm_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                + (f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}, double, true>(a, s);\n"
-                   if tag in GRAD_WAVE_MFMA else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
-                + "}\n"
-                f"hipError_t grad2e_launchwm32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                + (f"  return grad2e_launch_wave<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float, true>(a, s);\n"
-                   if tag in GRAD_WAVE_MFMA else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
-                + "}\n"
-                f"hipError_t grad2e_launcht_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}>(a, s);\n"
-                   if tag in GRAD_PER_THREAD else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
-                + "}\n"
-                f"hipError_t grad2e_launcht32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                + (f"  return grad2e_launch_thread<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float>(a, s);\n"
-                   if tag in GRAD_PER_THREAD_F32 else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
-                + "}\n"
-                f"hipError_t grad2e_launchp_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                + (f"  return grad2e_launch_prim<{c[0]}, {c[1]}, {c[2]}, {c[3]}, double, {ch[0]}>(a, s);\n"
-                   if tag in GRAD_PRIM else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
-                + "}\n"
-                f"hipError_t grad2e_launchp32_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
-                + (f"  return grad2e_launch_prim<{c[0]}, {c[1]}, {c[2]}, {c[3]}, float, {ch[1]}>(a, s);\n"
-                   if tag in GRAD_PRIM_F32 else "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n")
-                + "}\n}  // namespace qcmd\n")
-        if not os.path.exists(path) or open(path).read() != text:
-            with open(path, "w") as fh:
-                fh.write(text)
-        out.append((sum(c) * 2 + 1, path))
+        unsupported = "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n"
+
+        def fn(name, body):
+            return (f"hipError_t grad2e_{name}_{tag}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                    + body + "}\n")
+
+        args = f"{c[0]}, {c[1]}, {c[2]}, {c[3]}"
+        # one unit per kernel form, so that the heavy instantiations of one
+        # class compile in parallel
+        units = {
+            "": fn("launchw", f"  return grad2e_launch_wave<{args}>(a, s);\n")
+                + fn("launchw32", f"  return grad2e_launch_wave<{args}, float>(a, s);\n")
+                + fn("launcht", f"  return grad2e_launch_thread<{args}>(a, s);\n"
+                     if tag in GRAD_PER_THREAD else unsupported)
+                + fn("launcht32", f"  return grad2e_launch_thread<{args}, float>(a, s);\n"
+                     if tag in GRAD_PER_THREAD_F32 else unsupported),
+            "_p": fn("launchp", f"  return grad2e_launch_prim<{args}, double, {ch[0]}>(a, s);\n"
+                     if tag in GRAD_PRIM else unsupported)
+                + fn("launchp32", f"  return grad2e_launch_prim<{args}, float, {ch[1]}>(a, s);\n"
+                     if tag in GRAD_PRIM_F32 else unsupported),
+            "_m": fn("launchwm", f"  return grad2e_launch_wave<{args}, double, true>(a, s);\n"
+                     if tag in GRAD_WAVE_MFMA else unsupported)
+                + fn("launchwm32", f"  return grad2e_launch_wave<{args}, float, true>(a, s);\n"
+                     if tag in GRAD_WAVE_MFMA else unsupported),
+        }
+        for suffix, body in units.items():
+            path = os.path.join(GEN, f"grad2e_{tag}{suffix}.hip")
+            text = '#include "grad_prim.h"\nnamespace qcmd {\n' + body + "}  // namespace qcmd\n"
+            if not os.path.exists(path) or open(path).read() != text:
+                with open(path, "w") as fh:
+                    fh.write(text)
+            heavy = (suffix == "_p" and (tag in GRAD_PRIM or tag in GRAD_PRIM_F32)) or suffix == ""
+            out.append((sum(c) * 2 + (2 if heavy else 0), path))
     return out
 
 

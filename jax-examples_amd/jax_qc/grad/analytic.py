@@ -53,9 +53,9 @@ def grad_variant_for(cab: int, ccd: int, fp32: int = 0) -> int:
 THREAD_CAPABLE = {(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1), (4, 0), (3, 3)}
 # classes with the one-lane-per-primitive-quartet kernel (csrc/build.py
 # GRAD_PRIM, GRAD_PRIM_F32): (pp|pp), (ds|pp); in FP32 also (dp|ps), (dp|pp),
-# (dp|ds)
+# (dp|ds), (dp|dp), (dd|ps), (dd|pp)
 PRIM_CAPABLE = {(2, 2), (3, 2)}
-PRIM_CAPABLE_F32 = PRIM_CAPABLE | {(4, 1), (4, 2), (4, 3)}
+PRIM_CAPABLE_F32 = PRIM_CAPABLE | {(4, 1), (4, 2), (4, 3), (4, 4), (5, 1), (5, 2)}
 # classes with the one-wave-per-quartet kernel whose Y phase (G contracted
 # with the bra E products) runs on the matrix cores (variant 4; csrc/build.py
 # GRAD_WAVE_MFMA): (pp|pp), (ds|pp), (dp|ps), (dp|pp), (dp|dp)
