@@ -62,10 +62,11 @@ FP32_PEAK_TFLOPS = 157.3
 MEASURED_TRAFFIC = {"fp64": (1639021.6 * 2.0 + 54269.0) * 1024,
                     "mixed": (1152157.2 * 2.0 + 63845.3) * 1024}
 # the same for the K tile pass (symv_tiles_kernel<2,true,true,2>) of the
-# (H2O)_16 mixed workload: FETCH 12,877,282 KiB x 2.00 + WRITE 1,643,874 KiB
-# per launch (profiles/fock_symv_water16_uhf_mixed.md), 1.08 x the 26.0 GB
-# algorithmic pool bytes (the rest: column partials and the tile directory)
-TRAFFIC_C5_K = (12877282 * 2 + 1643874) * 1024
+# (H2O)_16 mixed workload (int8 / int16 / FP32 / FP64 tiles): FETCH 8,086,353
+# KiB x 2.00 + WRITE 1,643,874 KiB per launch (mean of 5 launches,
+# profiles/fock_and_energy_gradient_water16_uhf.md), 1.15 x the 15.9 GB algorithmic
+# pool bytes (the rest: column partials, x and the tile directory)
+TRAFFIC_C5_K = (8086353 * 2 + 1643874) * 1024
 
 
 def _cpu_info():
@@ -598,6 +599,36 @@ def _warm_eigh(n: int):
     torch.cuda.synchronize()
 
 
+def run_once(args):
+    """One (H2O)_16 UHF triplet energy + gradient on one path (after the same
+    untimed warm-up as run_water16), for whole-run profiler traces; prints
+    the stage wall times as one JSON line."""
+    import numpy as np
+    import torch
+
+    from jax_qc import molecules as M
+    from jax_qc.core.types import CalcConfig
+    from jax_qc.geomopt.optimizer import energy_and_gradient
+
+    torch.cuda.set_device(0)
+    opts = MIXED if args.once == "mixed" else STRICT
+    cfgc = CalcConfig(method="uhf", basis="even-tempered", scf_conv=1e-10, max_scf_iter=200)
+    w1 = M.water_cluster((1, 1, 1), seed=2, spin=2)
+    energy_and_gradient(np.asarray(w1.coords, dtype=np.float64), w1, "even-tempered", cfgc, **opts)
+    _warm_eigh(480)
+    mol = M.water_cluster((4, 2, 2), seed=2, spin=2)
+    tm = {}
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    E, g, _, res = energy_and_gradient(np.asarray(mol.coords, dtype=np.float64), mol,
+                                       "even-tempered", cfgc, timings=tm, **opts)
+    torch.cuda.synchronize()
+    print(json.dumps({"once": args.once, "energy": E, "wall_s": round(time.perf_counter() - t, 4),
+                      "scf_iterations": res.n_iterations,
+                      "stages_s": {k: round(v, 4) if isinstance(v, float) else v
+                                   for k, v in tm.items()}}), flush=True)
+
+
 def run_water16(args):
     """(H2O)_16 UHF triplet energy + gradient, FP64 vs mixed; mixed Fock
     builds/s as the headline value (see the module docstring)."""
@@ -765,6 +796,9 @@ def main():
                     help="also print per-class ERI kernel times and FLOP rates (one JSON line "
                          "per precision mode)")
     ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
+    ap.add_argument("--once", choices=["fp64", "mixed"], default=None,
+                    help="water16: one energy + gradient on that path (after the warm-up), "
+                         "stage times as JSON (for whole-run profiler traces)")
     ap.add_argument("--grad-report", action="store_true",
                     help="water16: per-launch derivative-ERI kernel times and FLOP rates of "
                          "both paths' gradients (JSON lines; no benchmark line)")
@@ -780,6 +814,8 @@ def main():
             return grad_report(args)
         if args.profile_steps:
             return profile_water16(args)
+        if args.once:
+            return run_once(args)
         return run_water16(args)
 
     import numpy as np
