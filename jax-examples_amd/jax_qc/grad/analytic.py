@@ -92,7 +92,7 @@ def _densities(mol, res):
 
 
 # Drop the quartets that the kernels' density screen would skip before the
-# launch (see prescreen_launch); the gradient is the same either way.
+# launch (see prescreen_launches); the gradient is the same either way.
 PRESCREEN = True
 
 
@@ -105,22 +105,13 @@ def _shell_block_max(M, pt, shell_of_cart):
     return out.reshape(ns, ns)
 
 
-def prescreen_launch(L, pt, shells, Mj, Mk, cx, Qd, tau_density):
-    """A copy of class launch L without the quartets (ab|cd) whose bound
-    (Mj[a,b] Mj[c,d] + cx sum_s (Mk_s[a,c] Mk_s[b,d] + Mk_s[a,d] Mk_s[b,c]))
-    * Q_ab Q_cd, with Mj, Mk_s the shell-block maxima of |Dj|, |Dk_s|, is
-    below tau_density / (1 + 1e-6): that bound is at least the max |G_abcd|
-    the kernels compute (in FP64 or FP32), so each quartet left out is one
-    the kernel's own density screen skips, and the gradient is unchanged --
-    only the waves that would have exited after forming G are not launched.
-    The surviving kets of each bra are listed explicitly (ket_list indexed
-    from kstart = the bra's offset)."""
-    from jax_qc.integrals.engine import ClassLaunch
-
+def _prescreen_mask(L, shells, Mj, Mk, cx, Qd, tau_density):
+    """(per-quartet bra index, ket pair, keep mask) of class launch L without
+    a host synchronisation (see prescreen_launches)."""
     dev = L.bra.device
     nq = L.nquartets
     cnt = (L.offsets[1:] - L.offsets[:-1])
-    bidx = torch.repeat_interleave(torch.arange(L.nbra, device=dev), cnt)
+    bidx = torch.repeat_interleave(torch.arange(L.nbra, device=dev), cnt, output_size=nq)
     local = torch.arange(nq, device=dev, dtype=torch.int64) - L.offsets[bidx]
     bp = L.bra.long()[bidx]
     kp = L.ket.long()[L.kstart.long()[bidx] + local]
@@ -130,14 +121,48 @@ def prescreen_launch(L, pt, shells, Mj, Mk, cx, Qd, tau_density):
     for M in Mk:
         bound = bound + cx * (M[a, c] * M[b, d] + M[a, d] * M[b, c])
     keep = bound * Qd[bp] * Qd[kp] * (1.0 + 1e-6) >= tau_density
-    del a, b, c, d, bound
-    kept = int(keep.sum())
-    ncnt = torch.bincount(bidx[keep], minlength=L.nbra)
-    offs = torch.zeros(L.nbra + 1, dtype=torch.int64, device=dev)
-    offs[1:] = torch.cumsum(ncnt, 0)
-    ket = kp[keep].to(torch.int32)
-    return ClassLaunch(L.cab, L.ccd, L.fp32, L.nbra, kept, L.bra, ket, offs,
-                       offs[:-1].to(torch.int32).contiguous())
+    return bidx, kp, keep
+
+
+def prescreen_launches(launches, pt, shells, Mj, Mk, cx, Qd, tau_density):
+    """Copies of the class launches without the quartets (ab|cd) whose bound
+    (Mj[a,b] Mj[c,d] + cx sum_s (Mk_s[a,c] Mk_s[b,d] + Mk_s[a,d] Mk_s[b,c]))
+    * Q_ab Q_cd, with Mj, Mk_s the shell-block maxima of |Dj|, |Dk_s|, is
+    below tau_density / (1 + 1e-6): that bound is at least the max |G_abcd|
+    the kernels compute (in FP64 or FP32), so each quartet left out is one
+    the kernel's own density screen skips, and the gradient is unchanged --
+    only the waves that would have exited after forming G are not launched.
+    The surviving kets of each bra are listed explicitly (ket_list indexed
+    from kstart = the bra's offset).  Every launch's mask is formed first and
+    their sizes read back together (one host synchronisation for the whole
+    gradient, not one per launch); the compaction then scatters the kept
+    quartets to their prefix-sum positions.  Launches left empty are
+    dropped."""
+    from jax_qc.integrals.engine import ClassLaunch
+
+    staged = [(L,) + _prescreen_mask(L, shells, Mj, Mk, cx, Qd, tau_density)
+              for L in launches if L.nquartets > 0]
+    if not staged:
+        return []
+    kept = torch.stack([k.sum() for (_, _, _, k) in staged]).cpu().tolist()
+    out = []
+    for (L, bidx, kp, keep), n in zip(staged, kept):
+        n = int(n)
+        if n == 0:
+            continue
+        dev = kp.device
+        pos = torch.cumsum(keep, 0) - 1
+        dst = torch.where(keep, pos, torch.full_like(pos, n))  # dropped ones -> slot n
+        ket = torch.empty(n + 1, dtype=torch.int64, device=dev).scatter_(0, dst, kp)[:n]
+        bra_of = torch.empty(n + 1, dtype=torch.int64, device=dev).scatter_(0, dst, bidx)[:n]
+        ncnt = torch.zeros(L.nbra, dtype=torch.int64, device=dev).scatter_add_(
+            0, bra_of, torch.ones(n, dtype=torch.int64, device=dev))
+        offs = torch.zeros(L.nbra + 1, dtype=torch.int64, device=dev)
+        offs[1:] = torch.cumsum(ncnt, 0)
+        out.append(ClassLaunch(L.cab, L.ccd, L.fp32, L.nbra, n, L.bra,
+                               ket.to(torch.int32).contiguous(), offs,
+                               offs[:-1].to(torch.int32).contiguous()))
+    return out
 
 
 def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_table=None,
@@ -184,11 +209,10 @@ def electronic_gradient(mol, basis, Dj, Dk, cx, W, tau: float = 1e-13, pair_tabl
                                              np.diff(pt.cart_off)).astype(np.int64)).to(dev)
             pre = (torch.from_numpy(pt.sp_shells.astype(np.int64)).to(dev),
                    _shell_block_max(Djc, pt, soc), [_shell_block_max(x, pt, soc) for x in Dkc])
-        for L in plan.launches:
-            if pre is not None:
-                L = prescreen_launch(L, pt, pre[0], pre[1], pre[2], cx, Qd, tau_density)
-                if L.nquartets == 0:
-                    continue
+        launches = plan.launches
+        if pre is not None:
+            launches = prescreen_launches(launches, pt, pre[0], pre[1], pre[2], cx, Qd, tau_density)
+        for L in launches:
             a = nat.QcGrad2eArgs()
             a.pairs64 = pt.pairs64.data_ptr()
             a.sp_prim_off = pt.sp_prim_off.data_ptr()

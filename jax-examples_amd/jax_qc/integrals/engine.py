@@ -113,6 +113,50 @@ def all_pair_records(sa, pa, pb, prim_cut: float = 0.0):
     return rec, cnt
 
 
+def pair_records_device(sa, pa, pb, dev, prim_cut: float = 0.0):
+    """``all_pair_records`` computed with torch on ``dev`` (the same
+    arithmetic, element for element; on the GPU the ~n_shell^2/2 x 9 rows cost
+    about a millisecond instead of ~0.1 s of host NumPy at 256 shells).
+    Returns the records as a (n, 12) FP64 tensor on ``dev`` and the per-pair
+    record counts (host)."""
+    f64 = dict(dtype=torch.float64, device=dev)
+    i64 = dict(dtype=torch.int64, device=dev)
+    ex = torch.as_tensor(np.asarray(sa["exps"], dtype=np.float64), **f64)
+    co = torch.as_tensor(np.asarray(sa["coefs"], dtype=np.float64), **f64)
+    ce = torch.as_tensor(np.asarray(sa["centers"], dtype=np.float64), **f64)
+    npr_h = np.asarray(sa["nprim"], dtype=np.int64)
+    off = torch.as_tensor(np.asarray(sa["prim_off"], dtype=np.int64), **i64)
+    npr = torch.as_tensor(npr_h, **i64)
+    cnt_h = npr_h[pa] * npr_h[pb]
+    total = int(cnt_h.sum())
+    pa_t, pb_t = torch.as_tensor(pa, **i64), torch.as_tensor(pb, **i64)
+    nb = npr[pb_t]
+    cnt = torch.as_tensor(cnt_h, **i64)
+    pid = torch.repeat_interleave(torch.arange(len(pa), **i64), cnt, output_size=total)
+    start = torch.cumsum(cnt, 0) - cnt
+    local = torch.arange(total, **i64) - start[pid]
+    ka = off[pa_t][pid] + torch.div(local, nb[pid], rounding_mode="floor")
+    kb = off[pb_t][pid] + torch.remainder(local, nb[pid])
+    al, be = ex[ka], ex[kb]
+    A, B = ce[pa_t][pid], ce[pb_t][pid]
+    p = al + be
+    P = (al[:, None] * A + be[:, None] * B) / p[:, None]
+    AB2 = torch.sum((A - B) ** 2, dim=1)
+    K = co[ka] * co[kb] * torch.exp(-al * be / p * AB2)
+    rec = torch.empty((total, 12), **f64)
+    rec[:, 0] = p
+    rec[:, 1:4] = P
+    rec[:, 4:7] = P - A
+    rec[:, 7:10] = P - B
+    rec[:, 10] = K
+    rec[:, 11] = al
+    if prim_cut > 0.0:
+        keep = K.abs() >= prim_cut
+        rec = rec[keep]
+        cnt_h = torch.bincount(pid[keep], minlength=len(pa)).cpu().numpy().astype(np.int64)
+    return rec.contiguous(), cnt_h
+
+
 class PairTable:
     """Shell-pair and primitive-pair tables for one basis on one device."""
 
@@ -125,7 +169,6 @@ class PairTable:
         self.basis = basis
         self.nbf = int(basis.n_basis)
         self.nshell = len(basis.shells)
-        self.dev = dev or nat.device()
         L, npr, off, ex, co, ce, ao = (sa[k] for k in ("l", "nprim", "prim_off", "exps", "coefs",
                                                         "centers", "ao_off"))
         ncl = (L + 1) * (L + 2) // 2
@@ -139,7 +182,8 @@ class PairTable:
         swap = L[ii] < L[jj]
         pa = np.where(swap, jj, ii).astype(np.int64)
         pb = np.where(swap, ii, jj).astype(np.int64)
-        rec, sp_n = all_pair_records(sa, pa, pb, prim_cut)
+        self.dev = dev or nat.device()
+        rec, sp_n = pair_records_device(sa, pa, pb, self.dev, prim_cut)
         sp_off = np.zeros(len(pa), dtype=np.int64)
         sp_off[1:] = np.cumsum(sp_n)[:-1]
         npp = int(sp_n.sum())
@@ -149,7 +193,7 @@ class PairTable:
         self.sp_ao_host = np.stack([ao[pa], ao[pb]], 1).astype(np.int32)
         self.nprim_pairs = npp
         d = self.dev
-        self.pairs64 = torch.from_numpy(np.ascontiguousarray(rec)).to(d)
+        self.pairs64 = rec
         self.pairs32 = self.pairs64.float() if fp32 else None
         self.sp_prim_off = torch.from_numpy(sp_off.astype(np.int32)).to(d)
         self.sp_nprim = torch.from_numpy(np.asarray(sp_n, dtype=np.int32)).to(d)
