@@ -260,11 +260,13 @@ typedef struct {
                           side contracted with G first, 3 one wave per
                           quartet with one lane per primitive quartet
                           ((pp|pp), (ds|pp); in FP32 also (dp|ps), (dp|pp),
-                          (dp|ds), (dp|dp), (dd|ps), (dd|pp)), 4 form 2
-                          with that contraction on the
+                          (dp|ds)), 4 form 2 with that contraction on the
                           matrix cores ((pp|pp), (ds|pp), (dp|ps), (dp|pp),
-                          (dp|dp)); a class without the form asked for
-                          returns an error */
+                          (dp|dp)), 5 one lane per primitive quartet with G
+                          contracted with the bra E first, per bra primitive
+                          in LDS ((pp|pp), (ds|pp), (dp|ps), (dp|pp), (dp|ds),
+                          (dp|dp), (dd|ps), (dd|pp): the default for them); a
+                          class without the form asked for returns an error */
   const double* schwarz; /* optional (with tau_density > 0): per shell pair Q */
   double tau_density;    /* skip a quartet when max|G_abcd| Q_ab Q_cd < tau_density */
   const float* pairs32;  /* fp32 = 1: FP32 primitive-pair table */

@@ -46,10 +46,9 @@ GRAD_PER_THREAD_F32 = GRAD_PER_THREAD | {"2200"}
 # FP32 (ds|pp) 134, (dp|pp) 233, (dp|ds) 229; FP64 (dp|ps) and (dd|ss) and
 # FP32 (dd|ps) spill 100-700 registers)
 GRAD_PRIM = {"1111", "2011"}
-# FP32 also (dp|ps), (dp|pp), (dp|ds) and, measured on the (H2O)_16 mixed
-# gradient against the wave form, (dp|dp) 79.5 -> 40.8 ms, (dd|pp) 25.7 ->
-# 18.3, (dd|ps) 22.0 -> 19.4 ((pp|ps) lost: 22.6 ms thread form, 246 ms)
-GRAD_PRIM_F32 = GRAD_PRIM | {"2110", "2111", "2120", "2121", "2211", "2210"}
+# FP32 also (dp|ps), (dp|pp), (dp|ds) (the library default now takes the
+# G-first form below for all of these; this form stays selectable as variant 3)
+GRAD_PRIM_F32 = GRAD_PRIM | {"2110", "2111", "2120"}
 # ket Cartesian pairs per launch of that kernel, (FP64, FP32), 1 where not
 # listed.  Measured on the (H2O)_16 gradient (1 -> chosen): FP32 (ds|pp)
 # 142.9 -> 21.8 ms at 9 (53.9 at 3), FP32 (pp|pp) 102.9 -> 39.4 ms at 3
@@ -57,6 +56,14 @@ GRAD_PRIM_F32 = GRAD_PRIM | {"2110", "2111", "2120", "2121", "2211", "2210"}
 # FP64 (ds|pp) (694 -> 876 at 9) and FP32 (dp|ps), (dp|pp), (dp|ds)
 # (105 -> 354, 92 -> 423, 41 -> 162 ms at 3)
 GRAD_PRIM_CHUNK = {"2011": (1, 9), "1111": (1, 3)}
+# derivative-ERI classes with the one-lane-per-primitive-quartet kernel that
+# contracts G with the bra E first (variant 5, grad_prim.h grad2e_yprim_kernel)
+# and their ket Cartesian pairs per launch (FP64, FP32): the LDS holds Y for
+# 9 bra primitives x the launch's ket pairs
+GRAD_YPRIM = {"1111", "2011", "2110", "2111", "2120", "2121", "2210", "2211"}
+GRAD_YPRIM_F32 = GRAD_YPRIM
+GRAD_YPRIM_CHUNK = {"1111": (3, 9), "2011": (3, 9), "2110": (3, 3), "2111": (3, 3),
+                    "2120": (3, 3), "2121": (3, 3), "2210": (1, 3), "2211": (1, 3)}
 # derivative-ERI classes that also get the wave kernel with its Y phase on the
 # matrix cores (variant 4; measured against the VALU Y phase)
 GRAD_WAVE_MFMA = {"1111", "2011", "2110", "2111", "2121"}
@@ -95,6 +102,7 @@ def _generate():
             # rough cost for scheduling: total angular momentum, fp64 first
             out.append((sum(c) * 2 + (prec == "f64"), path))
         ch = GRAD_PRIM_CHUNK.get(tag, (1, 1))
+        ych = GRAD_YPRIM_CHUNK.get(tag, (1, 1))
         unsupported = "  (void)a;\n  (void)s;\n  return hipErrorNotSupported;\n"
 
         def fn(name, body):
@@ -115,6 +123,10 @@ def _generate():
                      if tag in GRAD_PRIM else unsupported)
                 + fn("launchp32", f"  return grad2e_launch_prim<{args}, float, {ch[1]}>(a, s);\n"
                      if tag in GRAD_PRIM_F32 else unsupported),
+            "_y": fn("launchy", f"  return grad2e_launch_yprim<{args}, double, {ych[0]}>(a, s);\n"
+                     if tag in GRAD_YPRIM else unsupported)
+                + fn("launchy32", f"  return grad2e_launch_yprim<{args}, float, {ych[1]}>(a, s);\n"
+                     if tag in GRAD_YPRIM_F32 else unsupported),
             "_m": fn("launchwm", f"  return grad2e_launch_wave<{args}, double, true>(a, s);\n"
                      if tag in GRAD_WAVE_MFMA else unsupported)
                 + fn("launchwm32", f"  return grad2e_launch_wave<{args}, float, true>(a, s);\n"
@@ -126,7 +138,8 @@ def _generate():
             if not os.path.exists(path) or open(path).read() != text:
                 with open(path, "w") as fh:
                     fh.write(text)
-            heavy = (suffix == "_p" and (tag in GRAD_PRIM or tag in GRAD_PRIM_F32)) or suffix == ""
+            heavy = ((suffix == "_p" and (tag in GRAD_PRIM or tag in GRAD_PRIM_F32))
+                     or (suffix == "_y" and tag in GRAD_YPRIM) or suffix == "")
             out.append((sum(c) * 2 + (2 if heavy else 0), path))
     return out
 

@@ -186,7 +186,9 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
   hipError_t grad2e_launchp_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);    \
   hipError_t grad2e_launchp32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);  \
   hipError_t grad2e_launchwm_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);   \
-  hipError_t grad2e_launchwm32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
+  hipError_t grad2e_launchwm32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);  \
+  hipError_t grad2e_launchy_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);     \
+  hipError_t grad2e_launchy32_##A##B##C##D(const Grad2eArgs& a, hipStream_t s);
 QC_ERI_CLASSES(QC_GDECL)
 #undef QC_GDECL
 
@@ -260,7 +262,7 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   if (a.tau_density > 0.0 && !a.schwarz) return set_error("qc_grad2e: tau_density without schwarz");
   a.boys64 = boys_table64();
   if (!a.boys64) return set_error("qc_grad2e: qc_init not called on the current device");
-  if (in->variant < 0 || in->variant > 4) return set_error("qc_grad2e: variant %d", in->variant);
+  if (in->variant < 0 || in->variant > 5) return set_error("qc_grad2e: variant %d", in->variant);
   // default (measured on (H2O)_8 and (H2O)_16): one thread per quartet for
   // every class that has that kernel (ket (ss) up to (dp|ss), ket (ps) up to
   // (ds|ps), and (ds|ds)), one wave with the bra side contracted first for
@@ -268,18 +270,19 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   const bool has_thread = (ccd == 0 && cab <= 4) || (ccd == 1 && cab <= 3) || (cab == 3 && ccd == 3);
   // in FP32 the thread kernel also fits (dd|ss)
   const bool has_thread32 = has_thread || (cab == 5 && ccd == 0);
-  // one lane per primitive quartet where that form measured faster on
-  // (H2O)_16 (wave form -> per-primitive form): FP64 (pp|pp) 773 -> 453 ms,
-  // (ds|pp) 1183 -> 711 ms; FP32 (pp|pp) 121 -> 39 (three ket pairs per
-  // launch), (ds|pp) 142 -> 22 (all nine per launch), (dp|ps) 123 -> 106,
-  // (dp|pp) 111 -> 94, (dp|ds) 56 -> 41 ms
-  const bool prim64 = (cab == 2 && ccd == 2) || (cab == 3 && ccd == 2);
-  // (dp|dp) 79.5 -> 40.8, (dd|pp) 25.7 -> 18.3, (dd|ps) 22.0 -> 19.4 ms
-  const bool prim32 = (cab == 2 && ccd == 2) || (cab == 3 && ccd == 2) || (cab == 4 && ccd >= 1 && ccd <= 4) ||
-                      (cab == 5 && (ccd == 1 || ccd == 2));
+  // one lane per primitive quartet with G contracted with the bra E first
+  // (variant 5) for every class that has it: measured on (H2O)_16 against the
+  // previous per-class best (FP64: wave or one lane per primitive quartet
+  // without the G-first contraction): (dp|ps) 1074 -> 205 ms, (dp|pp) 1075 ->
+  // 296, (dp|dp) 753 -> 164, (dp|ds) 697 -> 116, (ds|pp) 695 -> 248, (pp|pp)
+  // 444 -> 157, (dd|ps) 250 -> 74, (dd|pp) 230 -> 94; FP32 (dp|ps) 106 -> 37,
+  // (dp|pp) 92 -> 37, (dp|dp) 41 -> 18, (pp|pp) 39 -> 16, (dp|ds) 41 -> 14
+  // ((pp|ps) lost: 5.3 -> 10.7 ms FP64, thread form kept)
+  const bool yprim = (cab == 2 && ccd == 2) || (cab == 3 && ccd == 2) || (cab == 4 && ccd >= 1) ||
+                     (cab == 5 && (ccd == 1 || ccd == 2));
   const int var = in->variant != 0
                       ? in->variant
-                      : ((in->fp32 ? has_thread32 : has_thread) ? 1 : ((in->fp32 ? prim32 : prim64) ? 3 : 2));
+                      : ((in->fp32 ? has_thread32 : has_thread) ? 1 : (yprim ? 5 : 2));
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipErrorInvalidValue;
 #define QC_GC(A, B, C, D)                                                                     \
@@ -287,6 +290,7 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
     e = var == 1 ? (in->fp32 ? grad2e_launcht32_##A##B##C##D(a, s) : grad2e_launcht_##A##B##C##D(a, s))  \
       : var == 3 ? (in->fp32 ? grad2e_launchp32_##A##B##C##D(a, s) : grad2e_launchp_##A##B##C##D(a, s))  \
       : var == 4 ? (in->fp32 ? grad2e_launchwm32_##A##B##C##D(a, s) : grad2e_launchwm_##A##B##C##D(a, s)) \
+      : var == 5 ? (in->fp32 ? grad2e_launchy32_##A##B##C##D(a, s) : grad2e_launchy_##A##B##C##D(a, s))   \
                  : (in->fp32 ? grad2e_launchw32_##A##B##C##D(a, s) : grad2e_launchw_##A##B##C##D(a, s));
   QC_ERI_CLASSES(QC_GC)
 #undef QC_GC
@@ -294,6 +298,7 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
     return set_error("qc_grad2e: class (%d, %d) has no %s kernel", cab, ccd,
                      var == 3   ? "one-lane-per-primitive-quartet"
                      : var == 4 ? "matrix-core Y-phase wave"
+                     : var == 5 ? "G-first one-lane-per-primitive-quartet"
                                 : "one-thread-per-quartet");
   return check(e, "qc_grad2e launch");
 }

@@ -25,7 +25,8 @@ from jax_qc.integrals import engine as eng
 # Kernel form per (bra class, ket class) of the derivative-ERI contraction:
 # 1 one thread per quartet, 2 one wave per quartet (bra side contracted with
 # G first), 3 one lane per primitive quartet, 4 form 2 with the G contraction
-# on the matrix cores, 0 the library's default.
+# on the matrix cores, 5 one lane per primitive quartet with G contracted with
+# the bra E first (the default where it exists), 0 the library's default.
 # Classes absent from the table use the default; a probe may set
 # GRAD_VARIANT_OVERRIDE to force one form for every class that has it.
 GRAD_VARIANTS: dict = {}
@@ -45,6 +46,8 @@ def grad_variant_for(cab: int, ccd: int, fp32: int = 0) -> int:
             return 2
         if ov == 4 and (cab, ccd) not in WAVE_MFMA_CAPABLE:
             return 2
+        if ov == 5 and (cab, ccd) not in YPRIM_CAPABLE:
+            return 2
         return ov
     return GRAD_VARIANTS.get((cab, ccd, int(fp32)), GRAD_VARIANTS.get((cab, ccd), 0))
 
@@ -53,13 +56,16 @@ def grad_variant_for(cab: int, ccd: int, fp32: int = 0) -> int:
 THREAD_CAPABLE = {(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1), (4, 0), (3, 3)}
 # classes with the one-lane-per-primitive-quartet kernel (csrc/build.py
 # GRAD_PRIM, GRAD_PRIM_F32): (pp|pp), (ds|pp); in FP32 also (dp|ps), (dp|pp),
-# (dp|ds), (dp|dp), (dd|ps), (dd|pp)
+# (dp|ds)
 PRIM_CAPABLE = {(2, 2), (3, 2)}
-PRIM_CAPABLE_F32 = PRIM_CAPABLE | {(4, 1), (4, 2), (4, 3), (4, 4), (5, 1), (5, 2)}
+PRIM_CAPABLE_F32 = PRIM_CAPABLE | {(4, 1), (4, 2), (4, 3)}
 # classes with the one-wave-per-quartet kernel whose Y phase (G contracted
 # with the bra E products) runs on the matrix cores (variant 4; csrc/build.py
 # GRAD_WAVE_MFMA): (pp|pp), (ds|pp), (dp|ps), (dp|pp), (dp|dp)
 WAVE_MFMA_CAPABLE = {(2, 2), (3, 2), (4, 1), (4, 2), (4, 4)}
+# classes with the one-lane-per-primitive-quartet kernel that contracts G with
+# the bra E first (variant 5; csrc/build.py GRAD_YPRIM)
+YPRIM_CAPABLE = {(2, 2), (3, 2), (4, 1), (4, 2), (4, 3), (4, 4), (5, 1), (5, 2)}
 
 
 def energy_weighted_density(C, orbital_energies, n_occ: int, factor: float = 2.0):

@@ -186,7 +186,7 @@ def test_uhf_water2_triplet_gradient_vs_full_scf_fd():
     pt = ints.ERI.pt
     forms = {}
     try:
-        for ov in (None, 1, 2, 3, 4):
+        for ov in (None, 1, 2, 3, 4, 5):
             an.GRAD_VARIANT_OVERRIDE = ov
             forms[ov] = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt).cpu().numpy()
             forms[(ov, "fp32")] = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt,
@@ -201,6 +201,7 @@ def test_uhf_water2_triplet_gradient_vs_full_scf_fd():
     np.testing.assert_allclose(forms[1], forms[2], atol=1e-11)
     np.testing.assert_allclose(forms[3], forms[2], atol=1e-11)
     np.testing.assert_allclose(forms[4], forms[2], atol=1e-11)
+    np.testing.assert_allclose(forms[5], forms[2], atol=1e-11)
 
 
 def _rigid_motion(coords, seed=7):

@@ -100,8 +100,8 @@ def test_water4_gradient_all_36_coordinates_vs_fd():
 
 def test_kernel_forms_fp32_split_and_density_screening():
     """The derivative-ERI kernel forms (one thread / one wave / one lane per
-    primitive quartet / one wave with the G contraction on the matrix cores)
-    agree to rounding; the FP32 split (Schwarz bound < 1e-4; either
+    primitive quartet / one wave with the G contraction on the matrix cores /
+    one lane per primitive quartet with G contracted first) agree to rounding; the FP32 split (Schwarz bound < 1e-4; either
     form's FP32 sums) and
     density-weighted screening (1e-11) stay within 1e-7 Eh/Bohr of FP64 on
     (H2O)_2 even-tempered (seed 1)."""
@@ -115,20 +115,19 @@ def test_kernel_forms_fp32_split_and_density_screening():
     pt = ints.ERI.pt
     g = {}
     try:
-        for ov in (1, 2, 3, 4):
+        for ov in (1, 2, 3, 4, 5):
             an.GRAD_VARIANT_OVERRIDE = ov
             g[ov] = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt).cpu().numpy()
     finally:
         an.GRAD_VARIANT_OVERRIDE = None
-    np.testing.assert_allclose(g[1], g[2], atol=1e-11)
-    np.testing.assert_allclose(g[3], g[2], atol=1e-11)
-    np.testing.assert_allclose(g[4], g[2], atol=1e-11)
+    for ov in (1, 3, 4, 5):
+        np.testing.assert_allclose(g[ov], g[2], atol=1e-11)
     g32 = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt, tau32=1e-4).cpu().numpy()
     print("max |fp32 split - fp64| =", np.abs(g32 - g[2]).max(), flush=True)
     np.testing.assert_allclose(g32, g[2], atol=1e-7)
     # the FP32 sums of each kernel form on their own
     try:
-        for ov in (1, 2, 3, 4):
+        for ov in (1, 2, 3, 4, 5):
             an.GRAD_VARIANT_OVERRIDE = ov
             g32f = an.scf_gradient(mol, bs, res, 1e-14, pair_table=pt, tau32=1e-4).cpu().numpy()
             print("form", ov, "max |fp32 split - fp64| =", np.abs(g32f - g[2]).max(), flush=True)
