@@ -60,10 +60,14 @@ GRAD_PRIM_CHUNK = {"2011": (1, 9), "1111": (1, 3)}
 # contracts G with the bra E first (variant 5, grad_prim.h grad2e_yprim_kernel)
 # and their ket Cartesian pairs per launch (FP64, FP32): the LDS holds Y for
 # 9 bra primitives x the launch's ket pairs
-GRAD_YPRIM = {"1111", "2011", "2110", "2111", "2120", "2121", "2210", "2211"}
-GRAD_YPRIM_F32 = GRAD_YPRIM
+GRAD_YPRIM = {"1111", "2011", "2110", "2111", "2120", "2121", "2200", "2210", "2211", "2220",
+              "2221"}
+GRAD_YPRIM_F32 = GRAD_YPRIM - {"2200"}
 GRAD_YPRIM_CHUNK = {"1111": (3, 9), "2011": (3, 9), "2110": (3, 3), "2111": (3, 3),
-                    "2120": (3, 3), "2121": (3, 3), "2210": (1, 3), "2211": (1, 3)}
+                    "2120": (3, 3), "2121": (3, 3), "2200": (1, 1), "2210": (1, 3),
+                    "2211": (1, 3), "2220": (1, 3), "2221": (1, 3)}
+# at most this many kernel instantiations (launch chunks) per translation unit
+YPRIM_CHUNKS_PER_UNIT = 3
 # derivative-ERI classes that also get the wave kernel with its Y phase on the
 # matrix cores (variant 4; measured against the VALU Y phase)
 GRAD_WAVE_MFMA = {"1111", "2011", "2110", "2111", "2121"}
@@ -110,6 +114,30 @@ def _generate():
                     + body + "}\n")
 
         args = f"{c[0]}, {c[1]}, {c[2]}, {c[3]}"
+        ncd = (c[2] + 1) * (c[2] + 2) // 2 * ((c[3] + 1) * (c[3] + 2) // 2)
+        part_units = {}
+
+        def yprim_entry(name, real, chunk, built):
+            """The G-first launcher of one precision: its chunks spread over
+            parts of at most YPRIM_CHUNKS_PER_UNIT instantiations, each part
+            its own translation unit, called in order by the entry point."""
+            if not built:
+                return fn(name, unsupported)
+            nch = -(-ncd // chunk)
+            per = YPRIM_CHUNKS_PER_UNIT
+            if nch <= per:
+                return fn(name, f"  return grad2e_launch_yprim<{args}, {real}, {chunk}>(a, s);\n")
+            body, decls = "", ""
+            for k, c0 in enumerate(range(0, nch, per)):
+                cda, cdb = c0 * chunk, min(ncd, (c0 + per) * chunk)
+                pname = f"{name}_{tag}_part{k}"
+                decls += f"hipError_t grad2e_{pname}(const Grad2eArgs& a, hipStream_t s);\n"
+                part_units[f"_y{real[0]}{k}"] = (
+                    f"hipError_t grad2e_{pname}(const Grad2eArgs& a, hipStream_t s) {{\n"
+                    f"  return grad2e_launch_yprim_range<{args}, {real}, {chunk}, {cda}, {cdb}>(a, s);\n}}\n")
+                body += f"  if (hipError_t e = grad2e_{pname}(a, s); e != hipSuccess) return e;\n"
+            return decls + fn(name, body + "  return hipSuccess;\n")
+
         # one unit per kernel form, so that the heavy instantiations of one
         # class compile in parallel
         units = {
@@ -123,15 +151,14 @@ def _generate():
                      if tag in GRAD_PRIM else unsupported)
                 + fn("launchp32", f"  return grad2e_launch_prim<{args}, float, {ch[1]}>(a, s);\n"
                      if tag in GRAD_PRIM_F32 else unsupported),
-            "_y": fn("launchy", f"  return grad2e_launch_yprim<{args}, double, {ych[0]}>(a, s);\n"
-                     if tag in GRAD_YPRIM else unsupported)
-                + fn("launchy32", f"  return grad2e_launch_yprim<{args}, float, {ych[1]}>(a, s);\n"
-                     if tag in GRAD_YPRIM_F32 else unsupported),
+            "_y": yprim_entry("launchy", "double", ych[0], tag in GRAD_YPRIM)
+                + yprim_entry("launchy32", "float", ych[1], tag in GRAD_YPRIM_F32),
             "_m": fn("launchwm", f"  return grad2e_launch_wave<{args}, double, true>(a, s);\n"
                      if tag in GRAD_WAVE_MFMA else unsupported)
                 + fn("launchwm32", f"  return grad2e_launch_wave<{args}, float, true>(a, s);\n"
                      if tag in GRAD_WAVE_MFMA else unsupported),
         }
+        units.update(part_units)
         for suffix, body in units.items():
             path = os.path.join(GEN, f"grad2e_{tag}{suffix}.hip")
             text = '#include "grad_prim.h"\nnamespace qcmd {\n' + body + "}  // namespace qcmd\n"
@@ -139,7 +166,7 @@ def _generate():
                 with open(path, "w") as fh:
                     fh.write(text)
             heavy = ((suffix == "_p" and (tag in GRAD_PRIM or tag in GRAD_PRIM_F32))
-                     or (suffix == "_y" and tag in GRAD_YPRIM) or suffix == "")
+                     or (suffix.startswith("_y") and tag in GRAD_YPRIM) or suffix == "")
             out.append((sum(c) * 2 + (2 if heavy else 0), path))
     return out
 

@@ -278,8 +278,10 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
   // 444 -> 157, (dd|ps) 250 -> 74, (dd|pp) 230 -> 94; FP32 (dp|ps) 106 -> 37,
   // (dp|pp) 92 -> 37, (dp|dp) 41 -> 18, (pp|pp) 39 -> 16, (dp|ds) 41 -> 14
   // ((pp|ps) lost: 5.3 -> 10.7 ms FP64, thread form kept)
+  // and (dd|ds) 135 -> 35 ms, (dd|dp) 294 -> 91 (FP32 14 -> 4, 30 -> 8),
+  // FP64 (dd|ss) 88 -> 20 (FP32 (dd|ss) keeps its thread form)
   const bool yprim = (cab == 2 && ccd == 2) || (cab == 3 && ccd == 2) || (cab == 4 && ccd >= 1) ||
-                     (cab == 5 && (ccd == 1 || ccd == 2));
+                     (cab == 5 && ccd <= 4);
   const int var = in->variant != 0
                       ? in->variant
                       : ((in->fp32 ? has_thread32 : has_thread) ? 1 : (yprim ? 5 : 2));

@@ -562,8 +562,14 @@ __global__ __launch_bounds__(64) void grad2e_yprim_kernel(Grad2eArgs args) {
             if constexpr (kz > 0) v -= (Real)kz * ekp[2 * W::EKS + ((kz - 1) * (LD + 1) + jz) * (LCD + 2) + vv];
             dz[vv] = v;
           });
-          // h0 over t+u+v <= la+lb+1, hC (C-derivative) over t+u+v <= la+lb
-          Real h0[NH1], hC[3][NH0];
+          // h0[k] (t+u+v <= la+lb+1) and hC_x,y,z[k] (t+u+v <= la+lb), each
+          // taken against Y as soon as it is formed (no h list is kept):
+          // dA_x: YA_x[k] h0[k]; dP_x: Y0[k - 1_x] h0[k] (the shift of R read
+          // from the other side); dC_x: Y0[k] hC_x[k]
+          const Real* __restrict__ ya0 = &ysh.ya[ib][0][cl][0];
+          const Real* __restrict__ ya1 = &ysh.ya[ib][1][cl][0];
+          const Real* __restrict__ ya2 = &ysh.ya[ib][2][cl][0];
+          const Real* __restrict__ y0p = &ysh.y0[ib][cl][0];
           sfor<0, NH1>([&](auto kI) {
             constexpr int k = decltype(kI)::value;
             constexpr TUV e = herm_decode(k);
@@ -589,32 +595,17 @@ __global__ __launch_bounds__(64) void grad2e_yprim_kernel(Grad2eArgs args) {
                 });
               });
             });
-            h0[k] = s0;
+            sm[0] += ya0[k] * s0;
+            sm[1] += ya1[k] * s0;
+            sm[2] += ya2[k] * s0;
+            if constexpr (e.t > 0) sm[3] += y0p[herm_index(e.t - 1, e.u, e.v)] * s0;
+            if constexpr (e.u > 0) sm[4] += y0p[herm_index(e.t, e.u - 1, e.v)] * s0;
+            if constexpr (e.v > 0) sm[5] += y0p[herm_index(e.t, e.u, e.v - 1)] * s0;
             if constexpr (low) {
-              hC[0][k] = cx;
-              hC[1][k] = cy;
-              hC[2][k] = cz;
-            }
-          });
-          // the h lists against Y of this lane's bra primitive
-          const Real* __restrict__ ya0 = &ysh.ya[ib][0][cl][0];
-          const Real* __restrict__ ya1 = &ysh.ya[ib][1][cl][0];
-          const Real* __restrict__ ya2 = &ysh.ya[ib][2][cl][0];
-          const Real* __restrict__ y0p = &ysh.y0[ib][cl][0];
-          sfor<0, NH1>([&](auto kI) {
-            constexpr int k = decltype(kI)::value;
-            sm[0] += ya0[k] * h0[k];
-            sm[1] += ya1[k] * h0[k];
-            sm[2] += ya2[k] * h0[k];
-            if constexpr (k < NH0) {
-              constexpr TUV e = herm_decode(k);
               const Real y = y0p[k];
-              sm[3] += y * h0[herm_index(e.t + 1, e.u, e.v)];
-              sm[4] += y * h0[herm_index(e.t, e.u + 1, e.v)];
-              sm[5] += y * h0[herm_index(e.t, e.u, e.v + 1)];
-              sm[6] += y * hC[0][k];
-              sm[7] += y * hC[1][k];
-              sm[8] += y * hC[2][k];
+              sm[6] += y * cx;
+              sm[7] += y * cy;
+              sm[8] += y * cz;
             }
           });
         });
@@ -659,6 +650,15 @@ hipError_t grad2e_launch_yprim(const Grad2eArgs& a, hipStream_t s) {
   if (a.nquartets <= 0) return hipSuccess;
   grad2e_launch_yprim_chunks<LA, LB, LC, LD, Real, 0, ncart(LC) * ncart(LD), CH>(
       grid_1d(a.nquartets, 64), a, s);
+  return hipGetLastError();
+}
+
+// The launches of ket pairs [CDA, CDB) only (CH per launch): the build
+// spreads the heavy classes' instantiations over several translation units.
+template <int LA, int LB, int LC, int LD, typename Real, int CH, int CDA, int CDB>
+hipError_t grad2e_launch_yprim_range(const Grad2eArgs& a, hipStream_t s) {
+  if (a.nquartets <= 0) return hipSuccess;
+  grad2e_launch_yprim_chunks<LA, LB, LC, LD, Real, CDA, CDB, CH>(grid_1d(a.nquartets, 64), a, s);
   return hipGetLastError();
 }
 

@@ -46,7 +46,7 @@ def grad_variant_for(cab: int, ccd: int, fp32: int = 0) -> int:
             return 2
         if ov == 4 and (cab, ccd) not in WAVE_MFMA_CAPABLE:
             return 2
-        if ov == 5 and (cab, ccd) not in YPRIM_CAPABLE:
+        if ov == 5 and (cab, ccd) not in (YPRIM_CAPABLE_F32 if fp32 else YPRIM_CAPABLE):
             return 2
         return ov
     return GRAD_VARIANTS.get((cab, ccd, int(fp32)), GRAD_VARIANTS.get((cab, ccd), 0))
@@ -65,7 +65,9 @@ PRIM_CAPABLE_F32 = PRIM_CAPABLE | {(4, 1), (4, 2), (4, 3)}
 WAVE_MFMA_CAPABLE = {(2, 2), (3, 2), (4, 1), (4, 2), (4, 4)}
 # classes with the one-lane-per-primitive-quartet kernel that contracts G with
 # the bra E first (variant 5; csrc/build.py GRAD_YPRIM)
-YPRIM_CAPABLE = {(2, 2), (3, 2), (4, 1), (4, 2), (4, 3), (4, 4), (5, 1), (5, 2)}
+YPRIM_CAPABLE = {(2, 2), (3, 2), (4, 1), (4, 2), (4, 3), (4, 4), (5, 0), (5, 1), (5, 2), (5, 3),
+                 (5, 4)}
+YPRIM_CAPABLE_F32 = YPRIM_CAPABLE - {(5, 0)}
 
 
 def energy_weighted_density(C, orbital_energies, n_occ: int, factor: float = 2.0):
