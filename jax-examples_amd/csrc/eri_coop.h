@@ -12,10 +12,13 @@
 // lanes and R^0 kept as a box so that index shifts add; per ket Cartesian
 // pair cd the list of its terms pref (-1)^{t'+u'+v'} E^{cd}_{t'u'v'};
 //   h_cd[tuv] = sum_{t'u'v'} c_{t'u'v'} R_{t+t',u+u',v+v'}   (t+u+v <= la+lb)
-// over all lanes; then V[ab][cd] += sum_{tuv} E^{ab}_{tuv} h_cd[tuv]
-// (obara_saika.py:285-331).  After all primitives the block goes through
-// LDS to the spherical basis (all lanes) and is written out like the
-// per-thread kernels.
+// summed over all ket primitives in registers (each thread owns fixed
+// (cd, tuv) entries); then once per bra primitive pair
+// V[ab][cd] += sum_{tuv} E^{ab}_{tuv} h_cd[tuv] (obara_saika.py:285-331;
+// E^{ab} does not depend on the ket primitive, so the bra contraction of the
+// ket-summed h equals the sum of the per-primitive ones).  After all
+// primitives the block goes through LDS to the spherical basis (all lanes)
+// and is written out like the per-thread kernels.
 #pragma once
 #include "eri_impl.h"
 
@@ -125,6 +128,9 @@ struct TermList {
 #ifndef QC_EW_KC_MIN
 #define QC_EW_KC_MIN 7
 #endif
+#ifndef QC_EW_WAVES
+#define QC_EW_WAVES 0  // > 0: that many waves per SIMD for every class (tuning sweeps)
+#endif
 constexpr int EW_BUDGET = 6144;                // 8-byte words of LDS per wave
 constexpr int EW_KC_BUDGET = QC_EW_KC_BUDGET;  // batching ket primitives stops here (occupancy)
 
@@ -133,9 +139,10 @@ constexpr int EW_KC_BUDGET = QC_EW_KC_BUDGET;  // batching ket primitives stops 
 // E^{ab}_{tuv} (coefficient, Hermite index).  Per batch of ket primitive
 // pairs: ket E tables, (-2 alpha)^n F_n, the R^0_tuv box (R recursion level
 // by level), per ket Cartesian pair cd the list of (pref (-1)^{t'+u'+v'}
-// E^{cd}_{t'u'v'}, box offset) terms, then h_cd[tuv] = sum_terms c R[tuv + t'u'v']
-// (items (cd, tuv) over all lanes), then every lane adds to the Cartesian
-// block elements it owns: V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv].
+// E^{cd}_{t'u'v'}, box offset) terms, then h_cd[tuv] += sum_terms c R[tuv + t'u'v']
+// (entries (cd, tuv) owned by fixed lanes, kept in registers).  After the
+// last batch the ket-summed h goes to LDS and every lane adds to the
+// Cartesian block elements it owns: V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv].
 // Both term lists are padded with zero terms to the class's largest count, so
 // the term loops have compile-time trip counts and issue all their LDS loads
 // back to back (a wave is latency-bound at one or two waves per SIMD).
@@ -155,12 +162,14 @@ struct EriWave {
   // measured slower ((dp|dp) 14.6 -> 17.6 ms, (dd|dp) 7.5 -> 9.7 ms)
   static constexpr int NT = LC + LD >= 4 ? 128 : 64;
   static constexpr int IPL = (NAB * NCD + NT - 1) / NT;  // block elements per thread
+  static constexpr int HPL = (NCD * NHB + NT - 1) / NT;  // h_cd[tuv] entries per thread
+  static constexpr int HS = NHB | 1;  // LDS row stride of the ket-summed h (odd)
   static constexpr int WR = sizeof(Real) == 8 ? 2 : 1;  // Real per 8-byte word, inverse
   // 8-byte words: fixed, per ket primitive of a batch
   static constexpr int FIX = (3 * EBS + NAB * TB) / (3 - WR) + (NAB * (TB + 1)) / 2 + NHB / 2 + 5;
-  static constexpr int KPER = (3 * EKS + 2 * NHR + RB * RB * RB + (L + 1) + 4 + NCD * TK +
-                               NCD * NHB) / (3 - WR) + (NCD * (TK + 1)) / 2 + 2;
-  static constexpr int BLK = NAB * NCD / (3 - WR) + 1;
+  static constexpr int KPER = (3 * EKS + 2 * NHR + RB * RB * RB + (L + 1) + 4 + NCD * TK) / (3 - WR) +
+                              (NCD * (TK + 1)) / 2 + 2;
+  static constexpr int BLK = (NAB * NCD > NCD * HS ? NAB * NCD : NCD * HS) / (3 - WR) + 1;
   static constexpr int lds(int kc) { return FIX + (kc * KPER > BLK ? kc * KPER : BLK); }
   static constexpr int pick_kc() {
     int kc = COOP_MAXP;
@@ -168,6 +177,13 @@ struct EriWave {
     return kc;
   }
   static constexpr int KC = pick_kc() >= QC_EW_KC_MIN ? pick_kc() : 1;
+  // waves per SIMD the register budget must allow: 3 for (dd|dp) FP64 (225
+  // VGPRs unconstrained, two waves; capped: (H2O)_16 48.4 -> 44.5 ms); every
+  // other class measured slower capped at 3 ((dd|dd) FP64 7.9 -> 10.1 ms,
+  // (dp|dp) 103 -> 108) and keeps the compiler's choice
+  static constexpr int OCC = QC_EW_WAVES > 0 ? QC_EW_WAVES
+                             : (LA == 2 && LB == 2 && LC == 2 && LD == 1 && sizeof(Real) == 8) ? 3
+                                                                                                : 1;
   struct PerKet {
     // first in LDS, so that a list's coefficients and offsets sit at small
     // immediate offsets from one base address; padded to TK terms with zero
@@ -178,26 +194,28 @@ struct EriWave {
     Real rc[KC][RB * RB * RB];
     Real f[KC][L + 1];
     Real misc[KC][4];  // X, Y, Z, pref
-    Real hb[KC][NCD][NHB];
   };
   struct Shared {
     union {
       PerKet k;
+      Real hs[NCD * HS];    // ket-summed h_cd[tuv], after a bra primitive's ket loop
       Real blk[NAB * NCD];  // the contracted Cartesian block, after the primitive loops
     } u;
     Real eb[3][EBS];
     TermList<TB, Real> bt[NAB];  // padded to TB terms with zero coefficients
+    Real pv[5];  // bra primitive pair: exponent, centre, prefactor
     int hbox[NHB];  // R-box offset of the (t, u, v) of Hermite index k
   };
 };
 
 template <int LA, int LB, int LC, int LD, typename Real>
-__global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT)) void eri_coop_kernel(EriArgs args) {
+__global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT))
+__attribute__((amdgpu_waves_per_eu(EriWave<LA, LB, LC, LD, Real>::OCC))) void eri_coop_kernel(EriArgs args) {
   using W = EriWave<LA, LB, LC, LD, Real>;
   constexpr int LAB = W::LAB, LCD = W::LCD, L = W::L;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int NAB = W::NAB, NCD = W::NCD, NHB = W::NHB, NHR = W::NHR, RB = W::RB;
-  constexpr int TB = W::TB, TK = W::TK, KC = W::KC, IPL = W::IPL;
+  constexpr int TB = W::TB, TK = W::TK, KC = W::KC, IPL = W::IPL, HPL = W::HPL, HS = W::HS;
   const HermTable<L>& HT = kHermTable<L>;
   __shared__ typename W::Shared sh;
   auto& K = sh.u.k;
@@ -212,9 +230,67 @@ __global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT)) void eri_coop_
   const int pb0 = args.sp_prim_off[bp], npb = args.sp_nprim[bp];
   const int pk0 = args.sp_prim_off[kp], npk = args.sp_nprim[kp];
 
+  // Per-thread constants of the loops below, read from the constant tables
+  // once: the recursion source of every R-level item the thread computes
+  // (packed dir | src1 << 2 | (src2 + 1) << 10 | mult << 19), the R-box
+  // offset of its level-0 items and the Cartesian exponents of its bra and
+  // ket term lists (packed 2 bits each).  Inside the primitive loops these
+  // lookups would each be a dependent global-memory round trip.
+  constexpr int RSL = (KC * NHR + NT - 1) / NT;  // R-level items per thread
+  constexpr int KSL = (KC * NCD + NT - 1) / NT;  // ket term lists per thread
+  constexpr int BSL = (NAB + NT - 1) / NT;       // bra term lists per thread
+  int prec[L + 1][RSL], rbx[RSL], kexp[KSL], bexp[BSL];
+#pragma unroll
+  for (int n = 0; n <= L; ++n)
+#pragma unroll
+    for (int i = 0; i < RSL; ++i) {
+      prec[n][i] = 0;
+      if (i * NT < KC * nherm(L - n)) {
+        const int k = (tid + NT * i) % nherm(L - n);
+        const HermRec hr = HT.rec[k];
+        prec[n][i] = hr.dir | ((hr.src1 & 255) << 2) | ((hr.src2 + 1) << 10) | (hr.mult << 19);
+      }
+    }
+#pragma unroll
+  for (int i = 0; i < RSL; ++i) {
+    const int tuv = kHermDecode<L>.tuv[(tid + NT * i) % NHR];
+    rbx[i] = ((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16);
+  }
+#pragma unroll
+  for (int i = 0; i < KSL; ++i) {
+    const int cd = (tid + NT * i) % NCD, c = cd / ND, d = cd - c * ND;
+    kexp[i] = CX[LC][c] | (CY[LC][c] << 2) | (CZ[LC][c] << 4) | (CX[LD][d] << 6) | (CY[LD][d] << 8) |
+              (CZ[LD][d] << 10);
+  }
+#pragma unroll
+  for (int i = 0; i < BSL; ++i) {
+    const int ab = (tid + NT * i) % NAB, a = ab / NB, b = ab - a * NB;
+    bexp[i] = CX[LA][a] | (CY[LA][a] << 2) | (CZ[LA][a] << 4) | (CX[LB][b] << 6) | (CY[LB][b] << 8) |
+              (CZ[LB][b] << 10);
+  }
   for (int k = tid; k < NHB; k += NT) {
     const int tuv = kHermDecode<LAB>.tuv[k];
     sh.hbox[k] = ((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16);
+  }
+  // Ket primitive-pair values a thread reads: the E lanes (tid < 3 KC: ket
+  // primitive tid / 3, direction tid % 3) PA, PB and the exponent; the Boys
+  // lanes (32 <= tid < 32 + KC) the exponent, the centre and the prefactor.
+  // (Loading them one batch ahead measured slower on (H2O)_16: the registers
+  // cost (dd|pp) FP64 a wave per SIMD, 42.6 -> 59.5 ms.)
+  static_assert(3 * KC <= 32, "E lanes and Boys lanes overlap");
+  const bool elane = tid < 3 * KC, blane = tid >= 32 && tid < 32 + KC;
+  const int qkk = elane ? tid / 3 : tid - 32, qdim = elane ? tid - 3 * (tid / 3) : 0;
+  int qf[5];
+  if (elane) {
+    qf[0] = PP_PAX + qdim;
+    qf[1] = PP_PBX + qdim;
+    qf[2] = qf[3] = qf[4] = PP_P;
+  } else {
+    qf[0] = PP_P;
+    qf[1] = PP_PX;
+    qf[2] = PP_PX + 1;
+    qf[3] = PP_PX + 2;
+    qf[4] = PP_K;
   }
   Real acc[IPL];
 #pragma unroll
@@ -223,6 +299,13 @@ __global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT)) void eri_coop_
   for (int ib = 0; ib < npb; ++ib) {
     const Real* P = pp + (int64_t)(pb0 + ib) * PP_STRIDE;
     __syncthreads();
+    if (tid == 3) {
+      sh.pv[0] = P[PP_P];
+      sh.pv[1] = P[PP_PX];
+      sh.pv[2] = P[PP_PX + 1];
+      sh.pv[3] = P[PP_PX + 2];
+      sh.pv[4] = P[PP_K];
+    }
     if (tid < 3) {
       Real E[LA + 1][LB + 1][LAB + 1];
       hermite_e<LA, LB, Real>(E, P[PP_PAX + tid], P[PP_PBX + tid], (Real)0.5 / P[PP_P]);
@@ -235,86 +318,100 @@ __global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT)) void eri_coop_
     }
     __syncthreads();
     // bra term lists
-    for (int ab = tid; ab < NAB; ab += NT) {
-      const int a = ab / NB, b = ab - a * NB;
-      const int ax = CX[LA][a], ay = CY[LA][a], az = CZ[LA][a];
-      const int bx = CX[LB][b], by = CY[LB][b], bz = CZ[LB][b];
-      int n = 0;
-      for (int t = 0; t <= ax + bx; ++t) {
-        const Real ex = eidx<LA, LB, Real>(sh.eb[0], ax, bx, t);
-        for (int u = 0; u <= ay + by; ++u) {
-          const Real exy = ex * eidx<LA, LB, Real>(sh.eb[1], ay, by, u);
-          for (int v = 0; v <= az + bz; ++v) {
-            sh.bt[ab].c[n] = exy * eidx<LA, LB, Real>(sh.eb[2], az, bz, v);
-            sh.bt[ab].d[n] = herm_index(t, u, v);
-            ++n;
+#pragma unroll
+    for (int i = 0; i < BSL; ++i) {
+      const int ab = tid + NT * i;
+      if (ab < NAB) {
+        int e = bexp[i];
+        asm volatile("" : "+v"(e));
+        const int ax = e & 3, ay = (e >> 2) & 3, az = (e >> 4) & 3;
+        const int bx = (e >> 6) & 3, by = (e >> 8) & 3, bz = (e >> 10) & 3;
+        int n = 0;
+        for (int t = 0; t <= ax + bx; ++t) {
+          const Real ex = eidx<LA, LB, Real>(sh.eb[0], ax, bx, t);
+          for (int u = 0; u <= ay + by; ++u) {
+            const Real exy = ex * eidx<LA, LB, Real>(sh.eb[1], ay, by, u);
+            for (int v = 0; v <= az + bz; ++v) {
+              sh.bt[ab].c[n] = exy * eidx<LA, LB, Real>(sh.eb[2], az, bz, v);
+              sh.bt[ab].d[n] = herm_index(t, u, v);
+              ++n;
+            }
           }
         }
-      }
-      for (; n < TB; ++n) {
-        sh.bt[ab].c[n] = (Real)0;
-        sh.bt[ab].d[n] = 0;
+        for (; n < TB; ++n) {
+          sh.bt[ab].c[n] = (Real)0;
+          sh.bt[ab].d[n] = 0;
+        }
       }
     }
+    Real hacc[HPL];
+#pragma unroll
+    for (int j = 0; j < HPL; ++j) hacc[j] = (Real)0;
     for (int ik0 = 0; ik0 < npk; ik0 += KC) {
       const int nk = min(KC, npk - ik0);
       __syncthreads();
-      if (tid < 3 * nk) {
-        const int kk = tid / 3, dim = tid - 3 * kk;
-        const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
+      Real qv[5];
+      if ((elane || blane) && qkk < nk) {
+        const Real* Q = pp + (int64_t)(pk0 + ik0 + qkk) * PP_STRIDE;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) qv[j] = Q[qf[j]];
+      }
+      if (elane && qkk < nk) {
         Real E[LC + 1][LD + 1][LCD + 1];
-        hermite_e<LC, LD, Real>(E, Q[PP_PAX + dim], Q[PP_PBX + dim], (Real)0.5 / Q[PP_P]);
+        hermite_e<LC, LD, Real>(E, qv[0], qv[1], (Real)0.5 / qv[2]);
 #pragma unroll
         for (int i = 0; i <= LC; ++i)
 #pragma unroll
           for (int j = 0; j <= LD; ++j)
 #pragma unroll
-            for (int t = 0; t <= LCD; ++t) K.ek[kk][dim][(i * (LD + 1) + j) * (LCD + 1) + t] = E[i][j][t];
+            for (int t = 0; t <= LCD; ++t) K.ek[qkk][qdim][(i * (LD + 1) + j) * (LCD + 1) + t] = E[i][j][t];
       }
-      if (tid >= 32 && tid - 32 < nk) {
-        const int kk = tid - 32;
-        const Real* Q = pp + (int64_t)(pk0 + ik0 + kk) * PP_STRIDE;
-        const Real p = P[PP_P], q = Q[PP_P];
+      if (blane && qkk < nk) {
+        const Real p = sh.pv[0], q = qv[0];
         const Real pqm = p * q, ps = p + q, alpha = pqm / ps;
-        const Real X = P[PP_PX] - Q[PP_PX], Y = P[PP_PX + 1] - Q[PP_PX + 1],
-                   Z = P[PP_PX + 2] - Q[PP_PX + 2];
+        const Real X = sh.pv[1] - qv[1], Y = sh.pv[2] - qv[2], Z = sh.pv[3] - qv[3];
         Real F[L + 1];
         boys_eval<L, Real>(alpha * (X * X + Y * Y + Z * Z), args.boys64, F);
         Real pw = (Real)1;
 #pragma unroll
         for (int n = 0; n <= L; ++n) {
-          K.f[kk][n] = pw * F[n];
+          K.f[qkk][n] = pw * F[n];
           pw *= (Real)-2 * alpha;
         }
-        K.misc[kk][0] = X;
-        K.misc[kk][1] = Y;
-        K.misc[kk][2] = Z;
-        K.misc[kk][3] = (Real)34.986836655249725693 / (pqm * dev_sqrt(ps)) * P[PP_K] * Q[PP_K];
+        K.misc[qkk][0] = X;
+        K.misc[qkk][1] = Y;
+        K.misc[qkk][2] = Z;
+        K.misc[qkk][3] = (Real)34.986836655249725693 / (pqm * dev_sqrt(ps)) * sh.pv[4] * qv[4];
       }
       __syncthreads();
       // ket term lists (pref and sign folded in)
-      for (int w = tid; w < nk * NCD; w += NT) {
-        const int kk = w / NCD, cd = w - kk * NCD;
-        const int c = cd / ND, d = cd - c * ND;
-        const int cx = CX[LC][c], cy = CY[LC][c], cz = CZ[LC][c];
-        const int dx = CX[LD][d], dy = CY[LD][d], dz = CZ[LD][d];
-        const Real pref = K.misc[kk][3];
-        int n = 0;
-        for (int tt = 0; tt <= cx + dx; ++tt) {
-          const Real ex = eidx<LC, LD, Real>(K.ek[kk][0], cx, dx, tt);
-          for (int uu = 0; uu <= cy + dy; ++uu) {
-            const Real exy = ex * eidx<LC, LD, Real>(K.ek[kk][1], cy, dy, uu);
-            for (int vv = 0; vv <= cz + dz; ++vv) {
-              const Real e = pref * exy * eidx<LC, LD, Real>(K.ek[kk][2], cz, dz, vv);
-              K.kt[kk][cd].c[n] = ((tt + uu + vv) & 1) ? -e : e;
-              K.kt[kk][cd].d[n] = (tt * RB + uu) * RB + vv;
-              ++n;
+#pragma unroll
+      for (int i = 0; i < KSL; ++i) {
+        const int w = tid + NT * i;
+        if (w < nk * NCD) {
+          const int kk = w / NCD, cd = w - kk * NCD;
+          int e = kexp[i];
+          asm volatile("" : "+v"(e));
+          const int cx = e & 3, cy = (e >> 2) & 3, cz = (e >> 4) & 3;
+          const int dx = (e >> 6) & 3, dy = (e >> 8) & 3, dz = (e >> 10) & 3;
+          const Real pref = K.misc[kk][3];
+          int n = 0;
+          for (int tt = 0; tt <= cx + dx; ++tt) {
+            const Real ex = eidx<LC, LD, Real>(K.ek[kk][0], cx, dx, tt);
+            for (int uu = 0; uu <= cy + dy; ++uu) {
+              const Real exy = ex * eidx<LC, LD, Real>(K.ek[kk][1], cy, dy, uu);
+              for (int vv = 0; vv <= cz + dz; ++vv) {
+                const Real e2 = pref * exy * eidx<LC, LD, Real>(K.ek[kk][2], cz, dz, vv);
+                K.kt[kk][cd].c[n] = ((tt + uu + vv) & 1) ? -e2 : e2;
+                K.kt[kk][cd].d[n] = (tt * RB + uu) * RB + vv;
+                ++n;
+              }
             }
           }
-        }
-        for (; n < TK; ++n) {
-          K.kt[kk][cd].c[n] = (Real)0;
-          K.kt[kk][cd].d[n] = 0;
+          for (; n < TK; ++n) {
+            K.kt[kk][cd].c[n] = (Real)0;
+            K.kt[kk][cd].d[n] = 0;
+          }
         }
       }
       // R^0_tuv (t+u+v <= L) per ket primitive of the batch, levels n = L
@@ -322,43 +419,58 @@ __global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT)) void eri_coop_
 #pragma unroll
       for (int n = L; n >= 0; --n) {
         const int nh = nherm(L - n);
-        for (int w = tid; w < nk * nh; w += NT) {
-          const int kk = w / nh, k = w - kk * nh;
-          Real* cur = K.r[n & 1][kk];
-          const Real* prv = K.r[(n + 1) & 1][kk];
-          Real v;
-          if (k == 0) {
-            v = K.f[kk][n];
-          } else {
-            const HermRec hr = HT.rec[k];
-            v = K.misc[kk][hr.dir] * prv[hr.src1];
-            if (hr.src2 >= 0) v += (Real)hr.mult * prv[hr.src2];
-          }
-          cur[k] = v;
-          if (n == 0) {
-            const int tuv = kHermDecode<L>.tuv[k];
-            K.rc[kk][((tuv & 255) * RB + ((tuv >> 8) & 255)) * RB + (tuv >> 16)] = v;
+#pragma unroll
+        for (int i = 0; i < RSL; ++i) {
+          const int w = tid + NT * i;
+          if (i * NT < KC * nh && w < nk * nh) {
+            const int kk = w / nh, k = w - kk * nh;
+            Real* cur = K.r[n & 1][kk];
+            const Real* prv = K.r[(n + 1) & 1][kk];
+            Real v;
+            if (k == 0) {
+              v = K.f[kk][n];
+            } else {
+              int pr = prec[n][i];
+              asm volatile("" : "+v"(pr));  // decode here, not hoisted out of the loops
+              v = K.misc[kk][pr & 3] * prv[(pr >> 2) & 255];
+              const int s2 = ((pr >> 10) & 511) - 1;
+              if (s2 >= 0) v += (Real)(pr >> 19) * prv[s2];
+            }
+            cur[k] = v;
+            if (n == 0) K.rc[kk][rbx[i]] = v;
           }
         }
         __syncthreads();
       }
-      // h_cd[tuv] for t+u+v <= la+lb
-      for (int w = tid; w < nk * NCD * NHB; w += NT) {
-        const int kk = w / (NCD * NHB), r0 = w - kk * NCD * NHB;
-        const int cd = r0 / NHB, k = r0 - cd * NHB;
-        K.hb[kk][cd][k] = term_sum<TK>(K.kt[kk][cd].c, K.kt[kk][cd].d, K.rc[kk] + sh.hbox[k]);
-      }
-      __syncthreads();
-      // V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv] for this thread's elements
+      // h_cd[tuv] (t+u+v <= la+lb) of this thread's entries, summed over the batch
 #pragma unroll
-      for (int j = 0; j < IPL; ++j) {
+      for (int j = 0; j < HPL; ++j) {
         const int it = tid + NT * j;
-        if (it < NAB * NCD) {
-          const int ab = it / NCD, cd = it - ab * NCD;
-          Real s = (Real)0;
-          for (int kk = 0; kk < nk; ++kk) s += term_sum<TB>(sh.bt[ab].c, sh.bt[ab].d, K.hb[kk][cd]);
-          acc[j] += s;
+        if (it < NCD * NHB) {
+          const int cd = it / NHB, k = it - cd * NHB;
+          const int hoff = sh.hbox[k];
+          for (int kk = 0; kk < nk; ++kk)
+            hacc[j] += term_sum<TK>(K.kt[kk][cd].c, K.kt[kk][cd].d, K.rc[kk] + hoff);
         }
+      }
+    }
+    __syncthreads();  // every lane is done with the last batch's lists and R boxes
+#pragma unroll
+    for (int j = 0; j < HPL; ++j) {
+      const int it = tid + NT * j;
+      if (it < NCD * NHB) {
+        const int cd = it / NHB, k = it - cd * NHB;
+        sh.u.hs[cd * HS + k] = hacc[j];
+      }
+    }
+    __syncthreads();
+    // V[ab][cd] += sum_terms E^{ab}_{tuv} h_cd[tuv] for this thread's elements
+#pragma unroll
+    for (int j = 0; j < IPL; ++j) {
+      const int it = tid + NT * j;
+      if (it < NAB * NCD) {
+        const int ab = it / NCD, cd = it - ab * NCD;
+        acc[j] += term_sum<TB>(sh.bt[ab].c, sh.bt[ab].d, sh.u.hs + cd * HS);
       }
     }
   }
@@ -388,6 +500,10 @@ __global__ __launch_bounds__((EriWave<LA, LB, LC, LD, Real>::NT)) void eri_coop_
   const int ao_c = args.sp_ao[2 * kp], ao_d = args.sp_ao[2 * kp + 1];
   const int64_t n = args.nbf;
   for (int o = tid; o < SA * SB * SC * SD; o += NT) {
+    // keeps the block's LDS reads inside the loop: hoisted, every lane would
+    // hold the whole Cartesian block in registers (216 doubles for (dd|ds)),
+    // which set the register count of the whole kernel
+    asm volatile("" ::: "memory");
     const int sa = o / (SB * SC * SD), sb = (o / (SC * SD)) % SB, sc = (o / SD) % SC, sd = o % SD;
     Real v = (Real)0;
     for (int a = 0; a < NA; ++a) {

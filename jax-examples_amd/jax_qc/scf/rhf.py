@@ -75,9 +75,10 @@ def run_rhf(mol: Molecule, ints: IntegralSet, config: CalcConfig = CalcConfig(),
         with optional_stage(timer, "initial_guess"):
             D0 = core_guess(H, X, n_occ)
             F0 = build_fock_rhf(H, D0, ints.ERI)
-            e0, C0 = _diagonalize_fock(F0, X)
             E0 = float(electronic_energy_rhf(D0, H, F0))
-        state = SCFState(density=D0, fock=F0, coefficients=C0, orbital_energies=e0, energy=E0,
+        # no orbitals yet: the first step replaces them, and a run that takes
+        # no step gets those of F0 from the final diagonalisation below
+        state = SCFState(density=D0, fock=F0, coefficients=None, orbital_energies=None, energy=E0,
                          iteration=0)
         hist = diis_history_init(S.shape[0], config.diis_space)
         converged, last = False, state.energy

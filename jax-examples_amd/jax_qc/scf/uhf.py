@@ -91,10 +91,10 @@ def run_uhf(mol: Molecule, ints: IntegralSet, config: CalcConfig = CalcConfig(),
         with optional_stage(timer, "orthogonalize"):
             X = symmetric_orthogonalization(S)
         with optional_stage(timer, "initial_guess"):
-            Da, Db, _ = core_guess_uhf(H, X, na, nb, ints.ERI)
+            Da, Db, _ = core_guess_uhf(H, X, na, nb, ints.ERI, orbitals=False)
             Fa, Fb = build_fock_uhf(H, Da, Db, ints.ERI)
-            (oa, Ca), (ob, Cb) = _diagonalize_pair(Fa, Fb, X)
-            Da, Db = density_uhf(Ca, Cb, na, nb)
+            Da, Db, orb = _density_pair_step(Fa, Fb, X, na, nb)
+            (oa, Ca), (ob, Cb) = orb if orb is not None else ((None, None), (None, None))
             E0 = float(electronic_energy_uhf(Da, Db, H, Fa, Fb))
         state = UHFState(Da, Db, Fa, Fb, Ca, Cb, oa, ob, E0, 0)
         hist = diis_history_init_uhf(S.shape[0], config.diis_space)

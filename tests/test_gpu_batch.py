@@ -60,7 +60,12 @@ def test_batched_rhf_energies_match_single_molecule_scf():
         ref = jax_qc.run_scf(m, ints, CalcConfig(method="rhf", scf_conv=1e-10))
         assert ref.converged
         assert abs(r.energies[i] - ref.energy) < 1e-8, (i, r.energies[i], ref.energy)
-        assert r.n_iterations[i] == ref.n_iterations
+        # same step rules, so the same iteration count, up to one step: the
+        # batched path sums its Fock builds, DIIS products and padded
+        # eigensolves in another order, and where |dE| lands next to the
+        # 1e-10 threshold those rounding differences decide the last step
+        assert abs(int(r.n_iterations[i]) - int(ref.n_iterations)) <= 1, (i, r.n_iterations[i],
+                                                                         ref.n_iterations)
 
 
 def test_batched_mixed_precision_within_a_microhartree():
