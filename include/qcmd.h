@@ -282,6 +282,25 @@ typedef struct {
  * 186-209 (which rebuilds all ERIs at R +- h e_k). */
 int qc_grad2e(int cab, int ccd, const qc_grad2e_args* args, void* stream);
 
+/* Launch-side density prescreen of one class launch's quartet list (bra i =
+ * pair bra[i] with kets ket[kstart[i] .. kstart[i] + offsets[i+1] - offsets[i]]).
+ * A quartet is kept when
+ *   (Mj[a,b] Mj[c,d] + cx sum_s (Mk_s[a,c] Mk_s[b,d] + Mk_s[a,d] Mk_s[b,c]))
+ *     * Q[bp] Q[kp] (1 + 1e-6) >= tau,
+ * (a, b) = shells[2 bp], shells[2 bp + 1] (likewise c, d of kp); Mj, Mk0 and
+ * the optional Mk1 are ns x ns row-major shell-block maxima of |Dj|, |Dk_s|.
+ * That bound is at least the max |G_abcd| qc_grad2e's own density screen
+ * computes, so the kept list gives the same gradient.  write_off == NULL:
+ * count[i] = kept kets of bra i.  Otherwise bra i's kept kets go, in list
+ * order, to out_ket[write_off[i] ...].  No counterpart in the reference (its
+ * gradient has no screening): it lets the mixed path launch only the
+ * quartets that survive (replacing a multi-pass torch formulation). */
+int qc_grad2e_prescreen(int nbra, const int64_t* offsets, const int* bra, const int* ket,
+                        const int* kstart, const int* shells, int ns, const double* Mj,
+                        const double* Mk0, const double* Mk1, double cx, const double* Q,
+                        double tau, int64_t* count, const int64_t* write_off, int* out_ket,
+                        void* stream);
+
 /* sum_ab (2 - delta_ij) [D_ab d(T+V)_ab/dR - W_ab dS_ab/dR] over the shell
  * pairs of class cab (including the nuclear-centre derivative of V), added
  * into grad.  Replaces the dH and dS terms of rhf_grad.py:186-209. */

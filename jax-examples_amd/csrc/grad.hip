@@ -196,6 +196,50 @@ QC_ERI_CLASSES(QC_GDECL)
 
 using namespace qcmd;
 
+// Launch-side density prescreen of one class launch's quartet list: one
+// wave per bra pair strides over its kets, keeps those whose bound
+//   (Mj[a,b] Mj[c,d] + cx sum_s (Mk_s[a,c] Mk_s[b,d] + Mk_s[a,d] Mk_s[b,c]))
+//     Q_ab Q_cd (1 + 1e-6) >= tau
+// (shell-block maxima of |Dj|, |Dk_s|; at least the max |G_abcd| the
+// gradient kernels' own screen computes, so no quartet they would keep is
+// dropped), and either counts them (count mode) or writes them, in list
+// order, from the bra's prefix-sum offset (a ballot per 64 kets).
+__global__ __launch_bounds__(256) void grad2e_prescreen_kernel(
+    int nbra, const int64_t* __restrict__ offsets, const int* __restrict__ bra,
+    const int* __restrict__ ket, const int* __restrict__ kstart, const int* __restrict__ shells,
+    int ns, const double* __restrict__ Mj, const double* __restrict__ Mk0,
+    const double* __restrict__ Mk1, double cx, const double* __restrict__ Q, double tau,
+    int64_t* __restrict__ count, const int64_t* __restrict__ write_off, int* __restrict__ out_ket) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= nbra) return;  // whole wave
+  const int bp = bra[i];
+  const int64_t a = shells[2 * bp], b = shells[2 * bp + 1];
+  const double qb = Q[bp] * (1.0 + 1e-6);
+  const double mjab = Mj[a * ns + b];
+  const int64_t n = offsets[i + 1] - offsets[i];
+  const int64_t k0 = kstart[i];
+  const int64_t base = write_off ? write_off[i] : 0;
+  int64_t kept = 0;
+  for (int64_t j0 = 0; j0 < n; j0 += 64) {
+    const int64_t j = j0 + lane;
+    bool keep = false;
+    int kp = 0;
+    if (j < n) {
+      kp = ket[k0 + j];
+      const int64_t c = shells[2 * kp], d = shells[2 * kp + 1];
+      double bound = mjab * Mj[c * ns + d];
+      bound += cx * (Mk0[a * ns + c] * Mk0[b * ns + d] + Mk0[a * ns + d] * Mk0[b * ns + c]);
+      if (Mk1) bound += cx * (Mk1[a * ns + c] * Mk1[b * ns + d] + Mk1[a * ns + d] * Mk1[b * ns + c]);
+      keep = bound * qb * Q[kp] >= tau;
+    }
+    const unsigned long long m = __ballot(keep);
+    if (out_ket && keep) out_ket[base + kept + __popcll(m & ((1ull << lane) - 1))] = kp;
+    kept += __popcll(m);
+  }
+  if (!out_ket && lane == 0) count[i] = kept;
+}
+
 extern "C" {
 
 int qc_grad1e(int cab, const double* pairs64, const int* sp_prim_off, const int* sp_nprim,
@@ -303,6 +347,24 @@ int qc_grad2e(int cab, int ccd, const qc_grad2e_args* in, void* stream) {
                      : var == 5 ? "G-first one-lane-per-primitive-quartet"
                                 : "one-thread-per-quartet");
   return check(e, "qc_grad2e launch");
+}
+
+int qc_grad2e_prescreen(int nbra, const int64_t* offsets, const int* bra, const int* ket,
+                        const int* kstart, const int* shells, int ns, const double* Mj,
+                        const double* Mk0, const double* Mk1, double cx, const double* Q,
+                        double tau, int64_t* count, const int64_t* write_off, int* out_ket,
+                        void* stream) {
+  if (nbra < 0 || ns <= 0) return set_error("qc_grad2e_prescreen: nbra %d, ns %d", nbra, ns);
+  if (!offsets || !bra || !ket || !kstart || !shells || !Mj || !Mk0 || !Q)
+    return set_error("qc_grad2e_prescreen: null input");
+  if (write_off ? !out_ket : !count)
+    return set_error("qc_grad2e_prescreen: need count (count mode) or write_off and out_ket");
+  if (nbra == 0) return 0;
+  const int per = 4;  // bra pairs (waves) per 256-thread block
+  hipLaunchKernelGGL(grad2e_prescreen_kernel, dim3((nbra + per - 1) / per), dim3(64 * per), 0,
+                     (hipStream_t)stream, nbra, offsets, bra, ket, kstart, shells, ns, Mj, Mk0,
+                     Mk1, cx, Q, tau, count, write_off, out_ket);
+  return check(hipGetLastError(), "qc_grad2e_prescreen launch");
 }
 
 }  // extern "C"

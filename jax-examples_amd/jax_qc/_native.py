@@ -77,6 +77,9 @@ _SIGS = {
     "qc_unpack_fock": [c_void_p, c_void_p, c_double, c_i64, c_int, c_void_p, c_void_p],
     "qc_dense_jk": [c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
     "qc_grad2e": [c_int, c_int, ctypes.POINTER(QcGrad2eArgs), c_void_p],
+    "qc_grad2e_prescreen": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                            c_void_p, c_void_p, c_void_p, ctypes.c_double, c_void_p,
+                            ctypes.c_double, c_void_p, c_void_p, c_void_p, c_void_p],
     "qc_grad1e": [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                   c_int, c_int, c_void_p, c_void_p, c_i64, c_void_p, c_void_p, c_void_p,
                   c_void_p],

@@ -113,25 +113,6 @@ def _shell_block_max(M, pt, shell_of_cart):
     return out.reshape(ns, ns)
 
 
-def _prescreen_mask(L, shells, Mj, Mk, cx, Qd, tau_density):
-    """(per-quartet bra index, ket pair, keep mask) of class launch L without
-    a host synchronisation (see prescreen_launches)."""
-    dev = L.bra.device
-    nq = L.nquartets
-    cnt = (L.offsets[1:] - L.offsets[:-1])
-    bidx = torch.repeat_interleave(torch.arange(L.nbra, device=dev), cnt, output_size=nq)
-    local = torch.arange(nq, device=dev, dtype=torch.int64) - L.offsets[bidx]
-    bp = L.bra.long()[bidx]
-    kp = L.ket.long()[L.kstart.long()[bidx] + local]
-    a, b = shells[bp, 0], shells[bp, 1]
-    c, d = shells[kp, 0], shells[kp, 1]
-    bound = Mj[a, b] * Mj[c, d]
-    for M in Mk:
-        bound = bound + cx * (M[a, c] * M[b, d] + M[a, d] * M[b, c])
-    keep = bound * Qd[bp] * Qd[kp] * (1.0 + 1e-6) >= tau_density
-    return bidx, kp, keep
-
-
 def prescreen_launches(launches, pt, shells, Mj, Mk, cx, Qd, tau_density):
     """Copies of the class launches without the quartets (ab|cd) whose bound
     (Mj[a,b] Mj[c,d] + cx sum_s (Mk_s[a,c] Mk_s[b,d] + Mk_s[a,d] Mk_s[b,c]))
@@ -140,35 +121,49 @@ def prescreen_launches(launches, pt, shells, Mj, Mk, cx, Qd, tau_density):
     the kernels compute (in FP64 or FP32), so each quartet left out is one
     the kernel's own density screen skips, and the gradient is unchanged --
     only the waves that would have exited after forming G are not launched.
-    The surviving kets of each bra are listed explicitly (ket_list indexed
-    from kstart = the bra's offset).  Every launch's mask is formed first and
-    their sizes read back together (one host synchronisation for the whole
-    gradient, not one per launch); the compaction then scatters the kept
-    quartets to their prefix-sum positions.  Launches left empty are
+    qc_grad2e_prescreen counts every launch's kept kets per bra (one wave per
+    bra pair); the counts are read back together (one host synchronisation
+    for the whole gradient), and a second pass writes the kept kets of each
+    bra, in list order, at its prefix-sum offset.  Launches left empty are
     dropped."""
     from jax_qc.integrals.engine import ClassLaunch
 
-    staged = [(L,) + _prescreen_mask(L, shells, Mj, Mk, cx, Qd, tau_density)
-              for L in launches if L.nquartets > 0]
+    staged = [L for L in launches if L.nquartets > 0]
     if not staged:
         return []
-    kept = torch.stack([k.sum() for (_, _, _, k) in staged]).cpu().tolist()
+    if not 1 <= len(Mk) <= 2:
+        raise ValueError(f"prescreen_launches: {len(Mk)} exchange densities (1 or 2 expected)")
+    dev = staged[0].bra.device
+    sh32 = shells.to(torch.int32).contiguous()
+    Mj = Mj.contiguous()
+    Mk = [m.contiguous() for m in Mk]
+    Mk1 = Mk[1] if len(Mk) > 1 else None
+    Qd = Qd.contiguous()
+    ns = int(Mj.shape[0])
+    s = nat.stream_ptr()
+
+    def run(L, count, write_off, out_ket):
+        nat.call("qc_grad2e_prescreen", L.nbra, nat.ptr(L.offsets), nat.ptr(L.bra), nat.ptr(L.ket),
+                 nat.ptr(L.kstart), nat.ptr(sh32), ns, nat.ptr(Mj), nat.ptr(Mk[0]), nat.ptr(Mk1),
+                 float(cx), nat.ptr(Qd), float(tau_density), nat.ptr(count), nat.ptr(write_off),
+                 nat.ptr(out_ket), s)
+
+    counts = []
+    for L in staged:
+        cnt = torch.empty(L.nbra, dtype=torch.int64, device=dev)
+        run(L, cnt, None, None)
+        counts.append(cnt)
+    kept = torch.stack([c.sum() for c in counts]).cpu().tolist()
     out = []
-    for (L, bidx, kp, keep), n in zip(staged, kept):
+    for L, cnt, n in zip(staged, counts, kept):
         n = int(n)
         if n == 0:
             continue
-        dev = kp.device
-        pos = torch.cumsum(keep, 0) - 1
-        dst = torch.where(keep, pos, torch.full_like(pos, n))  # dropped ones -> slot n
-        ket = torch.empty(n + 1, dtype=torch.int64, device=dev).scatter_(0, dst, kp)[:n]
-        bra_of = torch.empty(n + 1, dtype=torch.int64, device=dev).scatter_(0, dst, bidx)[:n]
-        ncnt = torch.zeros(L.nbra, dtype=torch.int64, device=dev).scatter_add_(
-            0, bra_of, torch.ones(n, dtype=torch.int64, device=dev))
         offs = torch.zeros(L.nbra + 1, dtype=torch.int64, device=dev)
-        offs[1:] = torch.cumsum(ncnt, 0)
-        out.append(ClassLaunch(L.cab, L.ccd, L.fp32, L.nbra, n, L.bra,
-                               ket.to(torch.int32).contiguous(), offs,
+        offs[1:] = torch.cumsum(cnt, 0)
+        ket = torch.empty(n, dtype=torch.int32, device=dev)
+        run(L, None, offs, ket)
+        out.append(ClassLaunch(L.cab, L.ccd, L.fp32, L.nbra, n, L.bra, ket, offs,
                                offs[:-1].to(torch.int32).contiguous()))
     return out
 

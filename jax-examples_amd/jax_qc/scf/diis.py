@@ -23,6 +23,9 @@ class DIISHistory:
     # ring: a push adds one row (a matrix-vector product) instead of a
     # (k x n^2)(n^2 x k) product, which BLAS runs as a single tile.
     gram: Optional[np.ndarray] = None
+    # max |e| of the last push (both spins for UHF), read back with the
+    # Gram row, so the drivers' convergence test needs no transfer of its own
+    err_max: float = float("nan")
 
     @property
     def size(self) -> int:
@@ -41,6 +44,8 @@ def diis_error(F, D, S, X):
 
 
 def _push(h: DIISHistory, F, e) -> DIISHistory:
+    """Append (F, e), evicting the oldest pair when full; the new Gram row
+    and max |e| come back to the host in one transfer."""
     if h.size == h.max_size:
         h.focks.pop(0)
         h.errors.pop(0)
@@ -49,7 +54,9 @@ def _push(h: DIISHistory, F, e) -> DIISHistory:
     h.focks.append(F)
     h.errors.append(e)
     v = e.reshape(-1)
-    row = torch.stack([torch.dot(x.reshape(-1), v) for x in h.errors]).cpu().numpy()
+    vals = torch.stack([torch.dot(x.reshape(-1), v) for x in h.errors] + [v.abs().max()])
+    vals = vals.cpu().numpy()
+    row, h.err_max = vals[:-1], float(vals[-1])
     m = len(h.errors)
     G = np.empty((m, m))
     if m > 1:

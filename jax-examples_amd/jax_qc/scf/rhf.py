@@ -86,7 +86,7 @@ def run_rhf(mol: Molecule, ints: IntegralSet, config: CalcConfig = CalcConfig(),
             state, hist, err = scf_step_rhf(state, H, ints.ERI, S, X, n_occ, hist,
                                             config.damping, timer)
             dE = abs(state.energy - last)
-            err_max = float(err.abs().max())
+            err_max = hist.err_max  # max |e| of this step's error, read back with DIIS
             last = state.energy
             if dE < config.scf_conv and err_max < max(config.scf_conv, 1e-6):
                 if not last_build_was_full(ints.ERI):

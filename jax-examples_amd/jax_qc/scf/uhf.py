@@ -78,7 +78,7 @@ def scf_step_uhf(state, H, ERI, S, X, n_alpha, n_beta, hist, damping, timer=None
     (oa, Ca), (ob, Cb) = orb if orb is not None else ((None, None), (None, None))
     with optional_stage(timer, "energy"):
         E = float(electronic_energy_uhf(Da, Db, H, Fa, Fb))
-    err = max(float(ea.abs().max()), float(eb.abs().max()))
+    err = hist.err_max  # max |e| over both spins (the stacked error of the last push)
     new = UHFState(Da, Db, Fa, Fb, Ca, Cb, oa, ob, E, state.iteration + 1)
     return new, hist, err
 

@@ -186,3 +186,67 @@ def test_density_prescreen_leaves_gradient_unchanged(method):
         finally:
             an.PRESCREEN = True
         np.testing.assert_allclose(g[True], g[False], atol=1e-13, rtol=0)
+
+
+@pytest.mark.parametrize("nk", [1, 2])
+def test_prescreen_kernel_keeps_exactly_the_bounded_quartets(nk):
+    """qc_grad2e_prescreen against the bound evaluated in NumPy for every
+    quartet of every launch ((H2O)_2 even-tempered, random shell-block
+    maxima spanning 1e-8..1, one or two exchange densities): the kept ket
+    lists, per launch and per bra in list order, are those with
+    bound * Q_ab Q_cd (1 + 1e-6) >= tau, apart from quartets within 1e-9
+    (relative) of the threshold, which may go either way."""
+    import jax_qc
+    from jax_qc.grad import analytic as an
+    from jax_qc.integrals import engine as eng
+
+    mol = M.water_cluster((2, 1, 1), seed=1)
+    bs = build_basis_set(mol, "even-tempered")
+    pt = eng.PairTable(bs, torch.device("cuda", 0))
+    plan = eng.plan_quartets(pt, 1e-12, 1e-4)
+    ns = pt.nshell
+    rng = np.random.default_rng(7)
+
+    def sym(x):
+        return np.maximum(x, x.T)
+
+    Mj = sym(10.0 ** rng.uniform(-8, 0, (ns, ns)))
+    Mk = [sym(10.0 ** rng.uniform(-8, 0, (ns, ns))) for _ in range(nk)]
+    cx, tau = 0.5, 1e-9
+    Q = pt.Q.astype(np.float64)
+    shells = pt.sp_shells.astype(np.int64)
+    dev = torch.device("cuda", 0)
+    got = an.prescreen_launches(plan.launches, pt, torch.from_numpy(shells).to(dev),
+                                torch.from_numpy(Mj).to(dev), [torch.from_numpy(m).to(dev) for m in Mk],
+                                cx, torch.from_numpy(Q).to(dev), tau)
+    got = {(L.cab, L.ccd, L.fp32): L for L in got}
+    n_checked = n_dropped = 0
+    for L in plan.launches:
+        bra = L.bra.cpu().numpy()
+        ket = L.ket.cpu().numpy()
+        offs = L.offsets.cpu().numpy()
+        ks = L.kstart.cpu().numpy()
+        G = got.get((L.cab, L.ccd, L.fp32))
+        if G is not None:
+            go, gk = G.offsets.cpu().numpy(), G.ket.cpu().numpy()
+        for i in range(L.nbra):
+            kp = ket[ks[i]:ks[i] + offs[i + 1] - offs[i]]
+            a, b = shells[bra[i]]
+            c, d = shells[kp, 0], shells[kp, 1]
+            bound = Mj[a, b] * Mj[c, d]
+            for m in Mk:
+                bound = bound + cx * (m[a, c] * m[b, d] + m[a, d] * m[b, c])
+            val = bound * Q[bra[i]] * Q[kp] * (1.0 + 1e-6)
+            sure = np.abs(val / tau - 1.0) > 1e-9
+            want = kp[(val >= tau) & sure]
+            if G is None:
+                assert not np.any((val >= tau) & sure), (L.cab, L.ccd, i)
+                continue
+            kept = gk[go[i]:go[i + 1]]
+            assert np.all(np.isin(want, kept)), (L.cab, L.ccd, L.fp32, i)
+            assert not np.any(np.isin(kp[(val < tau) & sure], kept)), (L.cab, L.ccd, L.fp32, i)
+            order = {k: j for j, k in enumerate(kp)}
+            assert all(order[x] < order[y] for x, y in zip(kept[:-1], kept[1:]))  # list order
+            n_checked += len(kp)
+            n_dropped += len(kp) - len(kept)
+    assert n_checked > 10000 and 0 < n_dropped < n_checked
