@@ -17,13 +17,14 @@ functions), energy + analytic gradient through two paths:
   accumulation, S^-1/2, eigensolve/purification and DIIS throughout.
 
 A *step* is one full UHF Fock build at the converged densities of the
-mixed path (pack -> J tile pass (1 right-hand side) -> reduce -> unpack,
-pack -> K tile pass (2) -> reduce -> unpack), i.e. what build_fock_uhf does
-per iteration without the incremental shortcut.  ``value`` = mixed Fock
+mixed path, F_a,b = H + P(Da + Db) -+ K(Da - Db)/2 with P = J - K/2 (pack ->
+P tile pass (1 right-hand side) -> reduce -> unpack, pack -> K tile pass
+(1) -> reduce -> unpack), i.e. what build_fock_uhf does per iteration
+without the incremental shortcut.  ``value`` = mixed Fock
 builds/s; the config carries both paths' Fock rates, energy + gradient wall
 times, the speed-up, |dE| and max |dgrad|.  The roofline is that of the
-dominant kernel of the mixed energy + gradient (the K tile pass,
-symv_tiles_kernel<2,true,*>, HBM-bound; algorithmic bytes = the compacted K
+dominant kernel of the mixed energy + gradient (the P tile pass,
+symv_tiles_kernel<1,true,*>, HBM-bound; algorithmic bytes = the compacted P
 pool per launch), timed with HIP events on the launch stream.
 cpu_baseline = the oracle's UHF Fock build (three np.einsum contractions of
 the reference's jax_qc/scf/fock.py:31-79) on a slab of a dense tensor,
@@ -61,12 +62,13 @@ FP32_PEAK_TFLOPS = 157.3
 # x1.00); profiles/symv_tiles_water8_*.md hold the commands and counts.
 MEASURED_TRAFFIC = {"fp64": (1639021.6 * 2.0 + 54269.0) * 1024,
                     "mixed": (1152157.2 * 2.0 + 63845.3) * 1024}
-# the same for the K tile pass (symv_tiles_kernel<2,true,true,2>) of the
-# (H2O)_16 mixed workload (int8 / int16 / FP32 / FP64 tiles): FETCH 8,086,353
-# KiB x 2.00 + WRITE 1,643,874 KiB per launch (mean of 5 launches,
-# profiles/fock_and_energy_gradient_water16_uhf.md), 1.15 x the 15.9 GB algorithmic
-# pool bytes (the rest: column partials, x and the tile directory)
-TRAFFIC_C5_K = (8086353 * 2 + 1643874) * 1024
+# the same for the P tile pass (symv_tiles_kernel<1,true,true,1> over the
+# compacted P = J - K/2 pool) of the (H2O)_16 mixed workload (int8 / int16 /
+# FP32 / FP64 tiles): FETCH 11,505,185 KiB x 2.00 + WRITE 856,173 KiB per
+# launch (mean of 5 launches, profiles/fock_and_energy_gradient_water16_uhf.md),
+# 1.06 x the 23.0 GB algorithmic pool bytes (the rest: column partials, x and
+# the tile directory)
+TRAFFIC_C5_P = (11505185 * 2 + 856173) * 1024
 
 
 def _cpu_info():
@@ -422,9 +424,12 @@ def run_batch(args):
 
 
 def _uhf_fock_rate(store, Da, Db, H, steps, warmup):
-    """Time `steps` full UHF Fock builds (F_s = H + J(Da + Db) - K(D_s)) on
-    the store's supermatrices; HIP events (torch's current stream, the launch
-    stream) bracket each J and K tile pass."""
+    """Time `steps` full UHF Fock builds on the store's supermatrices, as
+    ERIStore.fock_uhf makes them: with P assembled, F_a,b = H + P(Da + Db)
+    -+ K(Da - Db)/2 (a P pass and a K pass, one right-hand side each);
+    otherwise H + J(Da + Db) - K(D_s) (J with one right-hand side, K with
+    two).  HIP events (torch's current stream, the launch stream) bracket
+    each tile pass; keys "P"/"J" and "K"."""
     import numpy as np
     import torch
 
@@ -432,9 +437,11 @@ def _uhf_fock_rate(store, Da, Db, H, steps, warmup):
     from jax_qc.integrals.store import CompactSupermatrix
 
     n = store.nbf
-    J, K = store.J, store.ensure_K()
+    pk = store.P is not None
+    A, K = (store.P if pk else store.J), store.ensure_K()
+    a = "P" if pk else "J"
     s = nat.stream_ptr
-    ev = {"J": [], "K": []}
+    ev = {a: [], "K": []}
 
     def tile_pass(M, nrhs, x, work, xm):
         if isinstance(M, CompactSupermatrix):
@@ -452,16 +459,22 @@ def _uhf_fock_rate(store, Da, Db, H, steps, warmup):
             nat.call("qc_symv_reduce", n, nrhs, nat.ptr(y), nat.ptr(work), s())
 
     Dt = (Da + Db).contiguous()
-    Ds = torch.stack([Da, Db]).contiguous()
-    Fj = torch.empty_like(H)
-    Fk = torch.empty_like(Ds)
+    Fa_ = torch.empty_like(H)
     w1, x1, y1 = store._buffers(1)
-    w2, x2, y2 = store._buffers(2)
-    xm1, xm2 = store._xm[1], store._xm[2]
+    xm1 = store._xm[1]
+    if pk:
+        Dk = (Da - Db).contiguous()
+        Fk = torch.empty_like(H)
+        passes = ((a, A, 1, Dt, x1, y1, w1, xm1, H, Fa_, 1.0), ("K", K, 1, Dk, x1, y1, w1, xm1, None, Fk, 0.5))
+    else:
+        Dk = torch.stack([Da, Db]).contiguous()
+        Fk = torch.empty_like(Dk)
+        w2, x2, y2 = store._buffers(2)
+        passes = ((a, A, 1, Dt, x1, y1, w1, xm1, H, Fa_, 1.0),
+                  ("K", K, 2, Dk, x2, y2, w2, store._xm[2], None, Fk, 1.0))
 
     def step(record):
-        for name, M, nrhs, D, x, y, w, xm, Hp, F in (
-                ("J", J, 1, Dt, x1, y1, w1, xm1, H, Fj), ("K", K, 2, Ds, x2, y2, w2, xm2, None, Fk)):
+        for name, M, nrhs, D, x, y, w, xm, Hp, F, scale in passes:
             nat.call("qc_pack_density", nat.ptr(D), n, nrhs, nat.ptr(x), s())
             if record:
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -471,8 +484,10 @@ def _uhf_fock_rate(store, Da, Db, H, steps, warmup):
                 e1.record()
                 ev[name].append((e0, e1))
             reduce(M, nrhs, y, w, xm)
-            nat.call("qc_unpack_fock", nat.ptr(Hp), nat.ptr(y), 1.0, n, nrhs, nat.ptr(F), s())
-        return Fj - Fk[0], Fj - Fk[1]
+            nat.call("qc_unpack_fock", nat.ptr(Hp), nat.ptr(y), scale, n, nrhs, nat.ptr(F), s())
+        if pk:
+            return Fa_ - Fk, Fa_ + Fk
+        return Fa_ - Fk[0], Fa_ - Fk[1]
 
     for _ in range(warmup):
         step(False)
@@ -482,9 +497,9 @@ def _uhf_fock_rate(store, Da, Db, H, steps, warmup):
         Fa, Fb = step(True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t
-    ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
+    ms = {k: float(np.mean([x.elapsed_time(y) for x, y in v])) for k, v in ev.items()}
     nbytes = {k: (M.bytes if isinstance(M, CompactSupermatrix) else M.numel() * 8)
-              for k, M in (("J", J), ("K", K))}
+              for k, M in ((a, A), ("K", K))}
     return steps / dt, dt, ms, nbytes, (Fa, Fb)
 
 
@@ -512,7 +527,7 @@ def profile_water16(args):
     ints = jax_qc.compute_integrals(mol, bs, eri="store", tau=MIXED["tau"], tau32=MIXED["tau32"],
                                     prim_eps=MIXED["prim_eps"])
     st = ints.ERI
-    st.ensure_K()
+    st.assemble_p_k()  # P = J - K/2 and K: the pools the UHF builds use
     st.use_compact(tau_f32=args.tile_f32, tau_i16=MIXED["tile_i16"], tau_i8=MIXED["tile_i8"])
     n = bs.n_basis
     g = torch.Generator(device="cpu").manual_seed(0)
@@ -521,7 +536,7 @@ def profile_water16(args):
     rate, dt, ms, nbytes, _ = _uhf_fock_rate(st, A[0].contiguous(), A[1].contiguous(),
                                              ints.H_core.contiguous(), args.steps, args.warmup)
     print(json.dumps({"profile": "water16 mixed UHF Fock builds", "builds_per_s": rate,
-                      "J_ms": ms["J"], "K_ms": ms["K"], "J_bytes": nbytes["J"],
+                      "P_ms": ms["P"], "K_ms": ms["K"], "P_bytes": nbytes["P"],
                       "K_bytes": nbytes["K"]}), flush=True)
 
 
@@ -679,7 +694,7 @@ def run_water16(args):
         if mode == "mixed":
             P["tiles"] = {k: dict(zero=Mx.n_zero, i8=Mx.n_i8, i16=Mx.n_i16, f32=Mx.n_f32,
                                   f64=Mx.n_f64, bytes=Mx.bytes)
-                          for k, Mx in (("J", store.J), ("K", store.K))}
+                          for k, Mx in (("P", store.P), ("K", store.K))}
             if not args.no_cpu:
                 P["cpu_eri"] = cpu_eri_build(store.basis, store.pt.unscreened
                                              if hasattr(store.pt, "unscreened") else store.pt,
@@ -691,8 +706,8 @@ def run_water16(args):
     f, m = paths["fp64"], paths["mixed"]
     dE = abs(m["E"] - f["E"])
     dG = float(np.abs(m["g"] - f["g"]).max())
-    kms = m["ms"]["K"]
-    achieved = m["nbytes"]["K"] / (kms * 1e-3) / 1e9
+    pms = m["ms"]["P"]
+    achieved = m["nbytes"]["P"] / (pms * 1e-3) / 1e9
     steps_bfgs = []
     if args.bfgs_steps > 0:
         def cb(step, coords, E, g, res):
@@ -711,9 +726,10 @@ def run_water16(args):
                "scf_iterations": P["iters"], "converged": P["converged"],
                "fock_builds_per_s": round(P["rate"], 2),
                "fock_ms": round(P["dt"] / args.steps * 1e3, 4),
-               "J_pass_ms": round(P["ms"]["J"], 4), "K_pass_ms": round(P["ms"]["K"], 4),
-               "J_bytes": P["nbytes"]["J"], "K_bytes": P["nbytes"]["K"],
-               "J_pass_gbs": round(P["nbytes"]["J"] / (P["ms"]["J"] * 1e-3) / 1e9, 1),
+               "P_pass_ms": round(P["ms"]["P"], 4), "K_pass_ms": round(P["ms"]["K"], 4),
+               "P_bytes": P["nbytes"]["P"], "K_bytes": P["nbytes"]["K"],
+               "P_pass_gbs": round(P["nbytes"]["P"] / (P["ms"]["P"] * 1e-3) / 1e9, 1),
+               "K_pass_gbs": round(P["nbytes"]["K"] / (P["ms"]["K"] * 1e-3) / 1e9, 1),
                "eri_quartets_fp64": P["n64"], "eri_quartets_fp32": P["n32"],
                "eri_seconds": round(P["eri_s"], 4),
                "primitive_pairs_kept": round(P["prim_pairs_kept"], 4), "energy": P["E"]}
@@ -747,11 +763,11 @@ def run_water16(args):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": "symv_tiles_kernel<2,true,*> (K pass, compacted pool)",
-                     "bytes_per_launch": m["nbytes"]["K"], "avg_launch_ms": round(kms, 5)},
+                     "kernel": "symv_tiles_kernel<1,true,*> (P pass, compacted pool)",
+                     "bytes_per_launch": m["nbytes"]["P"], "avg_launch_ms": round(pms, 5)},
     }
-    if TRAFFIC_C5_K is not None:
-        out["roofline"]["traffic"] = TRAFFIC_C5_K
+    if TRAFFIC_C5_P is not None:
+        out["roofline"]["traffic"] = TRAFFIC_C5_P
         out["roofline"]["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated"
     if not args.no_cpu:
         cb = cpu_baseline(480, rows=2, spins=2)

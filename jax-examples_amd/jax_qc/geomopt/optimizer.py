@@ -65,7 +65,7 @@ def energy_and_gradient(coords, mol, basis_name, config, eri="store", tau=1e-12,
     ints = compute_integrals(m, bs, eri=eri, tau=tau, tau32=tau32, prim_eps=prim_eps)
     if eri == "store":
         if config.method.lower() == "uhf":
-            ints.ERI.ensure_K()
+            ints.ERI.assemble_p_k()  # UHF Fock builds from P and K (see ERIStore.fock_uhf)
         else:
             ints.ERI.ensure_P()
             ints.ERI.release_J()

@@ -139,6 +139,19 @@ class ERIStore:
                      nat.stream_ptr())
         return self.K
 
+    def assemble_p_k(self):
+        """K from J, then J turned into P = J - K/2 in place: the pair of
+        supermatrices the UHF Fock builds use (see fock_uhf), without a third
+        n_pair^2 buffer or a second gather pass.  Diagonal tiles hold both
+        triangles and the padding is zero in J and K alike, so this equals
+        the assembly of P from J element by element."""
+        K = self.ensure_K()
+        if self.P is None:
+            self._assemble_from_J("P")
+            self.J.add_(K, alpha=-0.5)
+            self.P, self.J = self.J, None
+        return self.P, K
+
     def release_J(self):
         """Drop the J supermatrix once only P is needed (RHF)."""
         self.ensure_P()
@@ -306,15 +319,24 @@ class ERIStore:
         return M
 
     def coulomb(self, D):
+        """J(D); from P and K (J = P + K/2) once J has been released."""
         if self.J is None:
-            raise RuntimeError("J supermatrix was released")
+            if self.P is None or self.K is None:
+                raise RuntimeError("J supermatrix was released")
+            return self.contract(self.P, D) + self.contract(self.K, D, scale=0.5)
         return self.contract(self.J, D)
 
     def exchange(self, D):
         return self.contract(self.ensure_K(), D)
 
     def fock_uhf(self, H, Da, Db):
-        """F_s = H + J(Da + Db) - K(D_s)."""
+        """F_s = H + J(Da + Db) - K(D_s).  With P = J - K/2 assembled, as
+        F_a,b = H + P(Da + Db) -+ K(Da - Db)/2 (the same matrices, K being
+        linear): two tile passes with one right-hand side each instead of
+        J with one and K with two, which the two-right-hand-side pass runs
+        at a lower share of HBM bandwidth."""
+        if self.P is not None and self.K is not None:
+            return self._fock_uhf_pk(H, Da, Db)
         if self.inc is not None:
             J = self._compact_or_raise(self.J)
             K = self._compact_or_raise(self.ensure_K())
@@ -329,6 +351,21 @@ class ERIStore:
         Jf = self.coulomb_plus(H, Da + Db)
         K = self.contract(self.ensure_K(), torch.stack([Da, Db]))
         return Jf - K[0], Jf - K[1]
+
+    def _fock_uhf_pk(self, H, Da, Db):
+        P, K = self.P, self.K
+        if self.inc is not None:
+            P, K = self._compact_or_raise(P), self._compact_or_raise(K)
+
+        def build(d, tau=0.0, Hd=None):
+            Gp = self.contract(P, d[0] + d[1], Hd, tau_skip=tau)
+            Gk = self.contract(K, d[0] - d[1], scale=0.5, tau_skip=tau)
+            return [Gp - Gk, Gp + Gk]
+
+        if self.inc is not None:
+            F = self._inc_fock(H, [Da, Db], build, lambda d: build(d, self.inc.tau))
+            return F[0], F[1]
+        return tuple(build([Da, Db], Hd=H))
 
     def coulomb_plus(self, H, D):
         if self.J is None:

@@ -141,6 +141,37 @@ def test_store_fock_matches_oracle_einsum(case):
     np.testing.assert_allclose(st.exchange(Dt).cpu().numpy(), scf.exchange(D, ref["ERI"]), atol=1e-11)
 
 
+def test_store_uhf_fock_from_p_and_k_matches_oracle(case):
+    """UHF Fock matrices from the J and K supermatrices and, after J is turned
+    into P = J - K/2 in place (assemble_p_k, the energy + gradient setup), from
+    P(Da + Db) -+ K(Da - Db)/2, both against the oracle's einsum build; J(D)
+    then comes from P + K/2, and P equals the gathered assembly of P from J."""
+    from jax_qc.integrals.store import ERIStore
+    from oracle import scf
+
+    mol, bs, ref = case
+    n = bs.n_basis
+    rng = np.random.default_rng(6)
+    Da, Db = [(A + A.T) / 2 for A in rng.normal(size=(2, n, n))]
+    want = scf.build_fock_uhf(ref["H_core"], Da, Db, ref["ERI"])
+    st = ERIStore(bs, tau=0.0)
+    dev = _dev()
+    Ht = torch.from_numpy(ref["H_core"]).to(dev)
+    Dat, Dbt = torch.from_numpy(Da).to(dev), torch.from_numpy(Db).to(dev)
+    for form in ("J and K", "P and K"):
+        if form == "P and K":
+            Pg = st.ensure_P().clone()  # gathered from J
+            st.P = None
+            st.assemble_p_k()
+            assert st.J is None and st.P is not None
+            np.testing.assert_allclose(st.P.cpu().numpy(), Pg.cpu().numpy(), atol=1e-15, rtol=1e-15)
+            del Pg
+        Fa, Fb = st.fock_uhf(Ht, Dat, Dbt)
+        np.testing.assert_allclose(Fa.cpu().numpy(), want[0], atol=1e-11, err_msg=form)
+        np.testing.assert_allclose(Fb.cpu().numpy(), want[1], atol=1e-11, err_msg=form)
+    np.testing.assert_allclose(st.coulomb(Dat).cpu().numpy(), scf.coulomb(Da, ref["ERI"]), atol=1e-11)
+
+
 def test_dense_jk_kernel(case):
     from jax_qc.scf.fock import _dense_jk
     from oracle import scf

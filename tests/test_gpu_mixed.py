@@ -163,7 +163,7 @@ def test_symv_two_wave_items_match_one_wave(dimer, nrhs):
         assert float((a - b).abs().max()) <= 1e-13 * scale
 
 
-@pytest.mark.parametrize("method", ["rhf", "uhf"])
+@pytest.mark.parametrize("method", ["rhf", "uhf", "uhf-pk"])
 def test_incremental_fock_builds_and_their_graphs(method):
     """Incremental Fock builds (difference densities, tiles screened at
     1e-13) land within 1e-8 Eh of full builds on the mixed store, the
@@ -172,15 +172,17 @@ def test_incremental_fock_builds_and_their_graphs(method):
     import jax_qc
     from jax_qc.integrals import store as stmod
 
-    spin = 2 if method == "uhf" else 0
+    spin = 2 if method.startswith("uhf") else 0
     mol = M.water_cluster((2, 1, 1), seed=1, spin=spin)
     bs = build_basis_set(mol, "even-tempered")
-    cfg = CalcConfig(method=method, scf_conv=1e-10, max_scf_iter=200)
+    cfg = CalcConfig(method=method[:3], scf_conv=1e-10, max_scf_iter=200)
 
     def run(inc, graphs):
         ints = jax_qc.compute_integrals(mol, bs, eri="store", tau=1e-12, tau32=1e-4)
-        if method == "uhf":
+        if method == "uhf":  # J and K pools
             ints.ERI.ensure_K()
+        elif method == "uhf-pk":  # P = J - K/2 and K pools (the energy + gradient setup)
+            ints.ERI.assemble_p_k()
         else:
             ints.ERI.ensure_P()
             ints.ERI.release_J()
