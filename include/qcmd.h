@@ -165,7 +165,10 @@ int qc_unpack_fock(const double* H, const double* y, double scale, int64_t nbf, 
  * is qc_tc2_partials(X) -> X2 = X X (any GEMM) -> qc_tc2_update(X, X2):
  * X_b <- X2_b if tr X_b > nocc[b], else 2 X_b - X2_b, the branch taken on
  * the device; idem[b] = |tr X_b - tr X_b^2| and (if trace is not NULL)
- * trace[b] = tr X_b of the matrix before the step.  part holds
+ * trace[b] = tr X_b of the matrix before the step; a matrix whose
+ * |tr X_b - tr X_b^2| is already below freeze is left unchanged (a
+ * converged X stays converged however many further steps run; 0 disables
+ * this).  part holds
  * qc_tc2_partials_len(nb) doubles of per-block partial sums.  No
  * allocation and no host synchronisation: a chunk of steps can be captured
  * in a graph.  Replaces the eigendecomposition of rhf.py:52-92 /
@@ -174,7 +177,7 @@ int qc_unpack_fock(const double* H, const double* y, double scale, int64_t nbf, 
 int64_t qc_tc2_partials_len(int nb);
 int qc_tc2_partials(const double* X, int64_t n, int nb, double* part, void* stream);
 int qc_tc2_update(double* X, const double* X2, const double* part, const double* nocc, int64_t n,
-                  int nb, double* idem, double* trace, void* stream);
+                  int nb, double* idem, double* trace, double freeze, void* stream);
 
 /* ---- many molecules per launch (batched SCF) -------------------------- */
 /* qc_one_electron for shell pairs of many molecules: pair sp writes its

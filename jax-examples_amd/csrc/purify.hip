@@ -63,15 +63,19 @@ __global__ __launch_bounds__(TC2_THREADS) void tc2_partials_kernel(const double*
 
 // Every block sums matrix b's partials itself (TC2_BLOCKS pairs, in a fixed
 // order, so every block takes the same branch), then writes its share of
-// X_b <- c1 X2_b + c2 X_b.  Block 0 also stores tr X_b and
-// idem[b] = |tr X_b - tr X_b^2| (the distance from idempotency) of the
-// matrix before the step.
+// X_b <- c1 X2_b + c2 X_b, unless |tr X_b - tr X_b^2| is already below
+// freeze: a converged X is left as it is (further products would amplify
+// its rounding -- an eigenvalue 1 + e becomes 1 + 2e under X^2, -e becomes
+// -2e under 2X - X^2 -- and diverge), so steps replayed past convergence are
+// harmless.  Block 0 also stores tr X_b and idem[b] = |tr X_b - tr X_b^2|
+// (the distance from idempotency) of the matrix before the step.
 __global__ __launch_bounds__(TC2_THREADS) void tc2_update_kernel(double* __restrict__ X,
                                                                  const double* __restrict__ X2,
                                                                  const double* __restrict__ part,
                                                                  const double* __restrict__ nocc,
                                                                  int64_t n, double* __restrict__ idem,
-                                                                 double* __restrict__ trace) {
+                                                                 double* __restrict__ trace,
+                                                                 double freeze) {
   const int b = blockIdx.y, blk = blockIdx.x;
   double tr = 0.0, fr = 0.0;
   for (int k = 0; k < TC2_BLOCKS; ++k) {
@@ -83,9 +87,10 @@ __global__ __launch_bounds__(TC2_THREADS) void tc2_update_kernel(double* __restr
   double* Xb = X + (int64_t)b * n * n;
   const double* X2b = X2 + (int64_t)b * n * n;
   const int64_t nn = n * n;
-  for (int64_t i = (int64_t)blk * TC2_THREADS + threadIdx.x; i < nn;
-       i += (int64_t)TC2_BLOCKS * TC2_THREADS)
-    Xb[i] = c1 * X2b[i] + c2 * Xb[i];
+  if (!(fabs(tr - fr) < freeze))
+    for (int64_t i = (int64_t)blk * TC2_THREADS + threadIdx.x; i < nn;
+         i += (int64_t)TC2_BLOCKS * TC2_THREADS)
+      Xb[i] = c1 * X2b[i] + c2 * Xb[i];
   if (blk == 0 && threadIdx.x == 0) {
     idem[b] = fabs(tr - fr);
     if (trace) trace[b] = tr;
@@ -110,12 +115,12 @@ int qc_tc2_partials(const double* X, int64_t n, int nb, double* part, void* stre
 }
 
 int qc_tc2_update(double* X, const double* X2, const double* part, const double* nocc, int64_t n,
-                  int nb, double* idem, double* trace, void* stream) {
+                  int nb, double* idem, double* trace, double freeze, void* stream) {
   if (n <= 0 || nb <= 0) return set_error("qc_tc2_update: n %lld, nb %d", (long long)n, nb);
   if (nb > 65535) return set_error("qc_tc2_update: nb %d above 65535", nb);
   if (!X || !X2 || !part || !nocc || !idem) return set_error("qc_tc2_update: null pointer");
   hipLaunchKernelGGL(tc2_update_kernel, dim3(TC2_BLOCKS, nb), dim3(TC2_THREADS), 0,
-                     (hipStream_t)stream, X, X2, part, nocc, n, idem, trace);
+                     (hipStream_t)stream, X, X2, part, nocc, n, idem, trace, freeze);
   return check(hipGetLastError(), "qc_tc2_update");
 }
 

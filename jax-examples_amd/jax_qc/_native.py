@@ -86,7 +86,7 @@ _SIGS = {
     "qc_tiles_maxabs": [c_void_p, c_i64, c_void_p, c_void_p],
     "qc_tc2_partials": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
     "qc_tc2_update": [c_void_p, c_void_p, c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p,
-                      c_void_p],
+                      c_double, c_void_p],
     "qc_one_electron_batch": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                               c_int, c_void_p, c_void_p, c_void_p, c_i64, c_void_p, c_void_p,
                               c_void_p],

@@ -37,7 +37,7 @@ class _TC2Graphs:
     """Captured TC2 start and step chunk for one (batch, n, occupations)
     shape; inputs and outputs are the static tensors F, X, idem."""
 
-    def __init__(self, B, n, n_occ, dev, dt, steps):
+    def __init__(self, B, n, n_occ, dev, dt, steps, tol):
         from jax_qc import _native as nat
 
         self.nat = nat
@@ -49,6 +49,7 @@ class _TC2Graphs:
         self.nocc = torch.tensor([float(k) for k in n_occ], dtype=dt, device=dev)
         self.eye = torch.eye(n, dtype=dt, device=dev)
         self.steps = steps
+        self.tol = float(tol)  # converged matrices stop changing (qc_tc2_update's freeze)
         self.B, self.n = B, n
         # chunks the last call needed: the next call replays all but one of
         # them before it first reads the convergence measure back
@@ -83,19 +84,19 @@ class _TC2Graphs:
             nat.call("qc_tc2_partials", nat.ptr(self.X), self.n, self.B, nat.ptr(self.part), s)
             torch.bmm(self.X, self.X, out=self.X2)
             nat.call("qc_tc2_update", nat.ptr(self.X), nat.ptr(self.X2), nat.ptr(self.part),
-                     nat.ptr(self.nocc), self.n, self.B, nat.ptr(self.idem), None, s)
+                     nat.ptr(self.nocc), self.n, self.B, nat.ptr(self.idem), None, self.tol, s)
 
 
-def _graphs_for(B, n, n_occ, dev, dt, steps):
+def _graphs_for(B, n, n_occ, dev, dt, steps, tol):
     global _GRAPH_CACHE
     from collections import OrderedDict
 
     if _GRAPH_CACHE is None:
         _GRAPH_CACHE = OrderedDict()
-    key = (B, n, tuple(int(k) for k in n_occ), str(dev), dt, steps)
+    key = (B, n, tuple(int(k) for k in n_occ), str(dev), dt, steps, float(tol))
     g = _GRAPH_CACHE.get(key)
     if g is None:
-        g = _TC2Graphs(B, n, n_occ, dev, dt, steps)
+        g = _TC2Graphs(B, n, n_occ, dev, dt, steps, tol)
         _GRAPH_CACHE[key] = g
         while len(_GRAPH_CACHE) > _GRAPH_CACHE_MAX:
             _GRAPH_CACHE.popitem(last=False)
@@ -123,7 +124,7 @@ def occupied_projectors(F: torch.Tensor, n_occ: Sequence[int], tol: float = 1e-1
                 return None
         return out
     if USE_GRAPHS and F.is_cuda and max_iter % check_every == 0 and dt == torch.float64:
-        g = _graphs_for(B, n, n_occ, dev, dt, check_every)
+        g = _graphs_for(B, n, n_occ, dev, dt, check_every, tol)
         g.F.copy_(F)
         g.g_start.replay()
         nchunk = max_iter // check_every
