@@ -179,6 +179,20 @@ int qc_tc2_partials(const double* X, int64_t n, int nb, double* part, void* stre
 int qc_tc2_update(double* X, const double* X2, const double* part, const double* nocc, int64_t n,
                   int nb, double* idem, double* trace, double freeze, void* stream);
 
+/* The same TC2 step in one launch: Y_b = X_b^2 if tr X_b > nocc[b], else
+ * 2 X_b - X_b^2 (X_b unchanged once |tr X_b - tr X_b^2| < freeze), with the
+ * product computed on FP64 matrix cores over 32 x 32 tiles of the lower
+ * triangle (X symmetric) and Y written in full.  part_in holds the per-tile
+ * partial sums (tr, |X|_F^2) of X -- from qc_tc2_tile_partials or from the
+ * previous step's part_out -- and part_out receives those of Y; both hold
+ * qc_tc2_tile_partials_len(n, nb) doubles.  idem[b] = |tr X_b - tr X_b^2|.
+ * X and Y, part_in and part_out must differ (steps ping-pong between two
+ * buffers).  No allocation, no host synchronisation (graph-capturable). */
+int64_t qc_tc2_tile_partials_len(int64_t n, int nb);
+int qc_tc2_tile_partials(const double* X, int64_t n, int nb, double* part, void* stream);
+int qc_tc2_step(const double* X, double* Y, const double* part_in, double* part_out,
+                const double* nocc, int64_t n, int nb, double* idem, double freeze, void* stream);
+
 /* ---- many molecules per launch (batched SCF) -------------------------- */
 /* qc_one_electron for shell pairs of many molecules: pair sp writes its
  * block at M + pair_base[sp] (row stride nmax, i.e. a padded

@@ -87,6 +87,9 @@ _SIGS = {
     "qc_tc2_partials": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
     "qc_tc2_update": [c_void_p, c_void_p, c_void_p, c_void_p, c_i64, c_int, c_void_p, c_void_p,
                       c_double, c_void_p],
+    "qc_tc2_tile_partials": [c_void_p, c_i64, c_int, c_void_p, c_void_p],
+    "qc_tc2_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64, c_int, c_void_p,
+                    c_double, c_void_p],
     "qc_one_electron_batch": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                               c_int, c_void_p, c_void_p, c_void_p, c_i64, c_void_p, c_void_p,
                               c_void_p],
@@ -128,6 +131,8 @@ def lib():
             L.qc_symv_workspace_bytes.argtypes = [c_i64, c_int]
             L.qc_tc2_partials_len.restype = c_i64
             L.qc_tc2_partials_len.argtypes = [c_int]
+            L.qc_tc2_tile_partials_len.restype = c_i64
+            L.qc_tc2_tile_partials_len.argtypes = [c_i64, c_int]
             L.qc_batch_items.restype = c_i64
             L.qc_batch_items.argtypes = [ctypes.POINTER(c_i64), c_int, c_int,
                                          ctypes.POINTER(c_i64)]
@@ -141,7 +146,7 @@ def lib():
 
 def exported_symbols():
     return (list(_SIGS) + ["qc_last_error", "qc_symv_workspace_bytes", "qc_batch_items",
-                           "qc_tc2_partials_len"]
+                           "qc_tc2_partials_len", "qc_tc2_tile_partials_len"]
             + list(_OPTIONAL_SIGS))
 
 

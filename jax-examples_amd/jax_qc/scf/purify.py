@@ -29,6 +29,11 @@ ENABLED = True
 # 2 x n^3 products, set the cost at a few hundred functions.  Off: every step
 # is launched eagerly with torch operations.
 USE_GRAPHS = True
+# On: each step is one qc_tc2_step launch (branch, FP64 matrix-core product
+# of the lower-triangle tiles, update and the next step's partial sums),
+# ping-ponging between two buffers; off: qc_tc2_partials, a batched library
+# GEMM and qc_tc2_update per step.
+FUSED_STEP = True
 _GRAPH_CACHE: "OrderedDict" = None
 _GRAPH_CACHE_MAX = 8
 
@@ -44,7 +49,13 @@ class _TC2Graphs:
         self.F = torch.zeros(B, n, n, dtype=dt, device=dev)
         self.X = torch.zeros_like(self.F)
         self.X2 = torch.zeros_like(self.F)
-        self.part = torch.zeros(int(nat.lib().qc_tc2_partials_len(B)), dtype=dt, device=dev)
+        self.fused = FUSED_STEP
+        if self.fused and steps % 2:
+            raise ValueError("fused TC2 steps ping-pong between two buffers: steps must be even")
+        plen = (nat.lib().qc_tc2_tile_partials_len(n, B) if self.fused
+                else nat.lib().qc_tc2_partials_len(B))
+        self.part = torch.zeros(int(plen), dtype=dt, device=dev)
+        self.part2 = torch.zeros_like(self.part)
         self.idem = torch.zeros(B, dtype=dt, device=dev)
         self.nocc = torch.tensor([float(k) for k in n_occ], dtype=dt, device=dev)
         self.eye = torch.eye(n, dtype=dt, device=dev)
@@ -77,9 +88,20 @@ class _TC2Graphs:
         lo = (d - r).min(dim=1).values
         hi = (d + r).max(dim=1).values
         self.X.copy_((hi[:, None, None] * self.eye - F) / (hi - lo)[:, None, None])
+        if self.fused:
+            self.nat.call("qc_tc2_tile_partials", self.nat.ptr(self.X), self.n, self.B,
+                          self.nat.ptr(self.part), self.nat.stream_ptr())
 
     def _chunk(self):
         nat, s = self.nat, self.nat.stream_ptr()
+        if self.fused:
+            # X -> X2 -> X ...: an even number of steps ends in X, partials in part
+            for i in range(self.steps):
+                src, dst = (self.X, self.X2) if i % 2 == 0 else (self.X2, self.X)
+                pin, pout = (self.part, self.part2) if i % 2 == 0 else (self.part2, self.part)
+                nat.call("qc_tc2_step", nat.ptr(src), nat.ptr(dst), nat.ptr(pin), nat.ptr(pout),
+                         nat.ptr(self.nocc), self.n, self.B, nat.ptr(self.idem), self.tol, s)
+            return
         for _ in range(self.steps):
             nat.call("qc_tc2_partials", nat.ptr(self.X), self.n, self.B, nat.ptr(self.part), s)
             torch.bmm(self.X, self.X, out=self.X2)
@@ -93,7 +115,7 @@ def _graphs_for(B, n, n_occ, dev, dt, steps, tol):
 
     if _GRAPH_CACHE is None:
         _GRAPH_CACHE = OrderedDict()
-    key = (B, n, tuple(int(k) for k in n_occ), str(dev), dt, steps, float(tol))
+    key = (B, n, tuple(int(k) for k in n_occ), str(dev), dt, steps, float(tol), FUSED_STEP)
     g = _GRAPH_CACHE.get(key)
     if g is None:
         g = _TC2Graphs(B, n, n_occ, dev, dt, steps, tol)

@@ -21,21 +21,28 @@ def _spectrum_matrix(n, n_occ, gap, seed):
     return 0.5 * (F + F.T), P
 
 
-def test_converged_matrices_stay_converged_across_replayed_chunks():
+@pytest.mark.parametrize("fused", [True, False], ids=["fused-step", "three-launch-step"])
+@pytest.mark.parametrize("n,n_occ", [(192, 70), (200, 81), (75, 30)])
+def test_converged_matrices_stay_converged_across_replayed_chunks(fused, n, n_occ):
     """A call that needs many step chunks (gap 1e-4 of the spectral width)
     followed by one that converges in a few (wide gap), same shapes: the
     second call replays the first call's chunk count before it reads the
-    convergence measure, i.e. steps past convergence; qc_tc2_update leaves a
-    converged matrix unchanged (without that, its rounding grows by 2x per
-    step and the projector is lost), so both projectors match eigh."""
+    convergence measure, i.e. steps past convergence; qc_tc2_update and
+    qc_tc2_step leave a converged matrix unchanged (without that, its
+    rounding grows by 2x per step and the projector is lost), so every
+    projector matches eigh.  Both step forms; n = 200 and 75 leave partial
+    32 x 32 tiles (75: odd rows, no 16-byte loads)."""
     from jax_qc.scf import purify
 
-    n, n_occ = 192, 70
     dev = torch.device("cuda", 0)
-    for gap, seed in ((1e-4, 1), (0.5, 2), (1e-4, 3), (0.3, 4)):
-        F, P = _spectrum_matrix(n, n_occ, gap, seed)
-        F2 = torch.stack([F, F.flip(0).flip(1)]).to(dev)  # the second: same spectrum, permuted basis
-        P2 = torch.stack([P, P.flip(0).flip(1)])
-        got = purify.occupied_projectors(F2, [n_occ, n_occ])
-        assert got is not None, gap
-        assert float((got.cpu() - P2).abs().max()) < 1e-9, gap
+    purify.FUSED_STEP = fused
+    try:
+        for gap, seed in ((1e-4, 1), (0.5, 2), (1e-4, 3), (0.3, 4)):
+            F, P = _spectrum_matrix(n, n_occ, gap, seed)
+            F2 = torch.stack([F, F.flip(0).flip(1)]).to(dev)  # same spectrum, permuted basis
+            P2 = torch.stack([P, P.flip(0).flip(1)])
+            got = purify.occupied_projectors(F2, [n_occ, n_occ])
+            assert got is not None, gap
+            assert float((got.cpu() - P2).abs().max()) < 1e-9, gap
+    finally:
+        purify.FUSED_STEP = True
