@@ -371,20 +371,42 @@ def plan_quartets(pt: PairTable, tau: float = 1e-12, tau32: float | None = None)
                     continue
                 offs = np.zeros(len(bl) + 1, dtype=np.int64)
                 offs[1:] = np.cumsum(cnt)
-                d = pt.dev
+                # host arrays for now; moved to the device together below
                 plan.launches.append(ClassLaunch(
-                    cab, ccd, fp32, len(bl), n,
-                    torch.from_numpy(np.ascontiguousarray(bl)).to(d),
-                    torch.from_numpy(np.ascontiguousarray(kl)).to(d),
-                    torch.from_numpy(offs).to(d),
-                    torch.from_numpy(start.astype(np.int32)).to(d)))
+                    cab, ccd, fp32, len(bl), n, np.ascontiguousarray(bl, dtype=np.int32),
+                    np.ascontiguousarray(kl, dtype=np.int32), offs, start.astype(np.int32)))
                 plan.counts[(cab, ccd, fp32)] = n
                 if fp32:
                     plan.n32 += n
                 else:
                     plan.n64 += n
     plan.n_screened = total - plan.n64 - plan.n32
+    _launches_to_device(plan.launches, pt.dev)
     return plan
+
+
+def _launches_to_device(launches, dev) -> None:
+    """Replace the launches' host index arrays by contiguous slices of two
+    device buffers (one int32, one int64): two host-to-device copies for the
+    whole plan instead of four per launch."""
+    if not launches:
+        return
+    i32 = [a for L in launches for a in (L.bra, L.ket, L.kstart)]
+    n32 = [len(a) for a in i32]
+    b32 = torch.from_numpy(np.concatenate(i32).astype(np.int32)).to(dev)
+    b64 = torch.from_numpy(np.concatenate([L.offsets for L in launches])).to(dev)
+    p32 = p64 = 0
+    k = 0
+    for L in launches:
+        views = []
+        for _ in range(3):
+            views.append(b32[p32:p32 + n32[k]])
+            p32 += n32[k]
+            k += 1
+        L.bra, L.ket, L.kstart = views
+        m = len(L.offsets)
+        L.offsets = b64[p64:p64 + m]
+        p64 += m
 
 
 def run_eri(pt: PairTable, plan: QuartetPlan, mode: int, out: torch.Tensor,

@@ -63,10 +63,14 @@ class CompactSupermatrix:
 
 def tile_formats(tm: np.ndarray, tau_f32: float, tau_zero: float = 1e-14,
                  tau_i16: float = 0.0, tau_i8: float = 0.0) -> np.ndarray:
-    """Per-tile format codes from the tiles' max |element| tm: zero below
+    """Per-tile format codes from the tiles' max |element| tm (a NumPy
+    array or a device tensor; the result is of the same kind): zero below
     tau_zero, int8 below tau_i8, int16 below tau_i16, FP32 below tau_f32,
     FP64 otherwise."""
-    fmt = np.full(tm.shape, TILE_F64, dtype=np.uint8)
+    if isinstance(tm, torch.Tensor):
+        fmt = torch.full(tm.shape, TILE_F64, dtype=torch.uint8, device=tm.device)
+    else:
+        fmt = np.full(tm.shape, TILE_F64, dtype=np.uint8)
     fmt[tm < tau_f32] = TILE_F32
     fmt[tm < tau_i16] = TILE_I16
     fmt[tm < tau_i8] = TILE_I8
@@ -168,22 +172,21 @@ class ERIStore:
         nt = T * (T + 1) // 2
         tmax = torch.empty(nt, dtype=torch.float64, device=self.dev)
         nat.call("qc_sm_tile_maxabs", nat.ptr(M), self.nbf, nat.ptr(tmax), nat.stream_ptr())
-        tm = tmax.cpu().numpy()
-        fmt = tile_formats(tm, tau_f32, tau_zero, tau_i16, tau_i8)
-        size = np.zeros(nt, dtype=np.int64)
-        for f, b in TILE_BYTES.items():
-            size[fmt == f] = b
-        off = np.zeros(nt, dtype=np.int64)
-        off[1:] = np.cumsum(size)[:-1]
-        total = int(size.sum())
+        # formats, sizes and pool offsets on the device; only the five format
+        # counts come back to the host (to size the pool)
+        fd = tile_formats(tmax, tau_f32, tau_zero, tau_i16, tau_i8)
+        nbytes = torch.tensor([TILE_BYTES[f] for f in range(len(TILE_BYTES))], dtype=torch.int64,
+                              device=self.dev)
+        size = nbytes[fd.long()]
+        od = (torch.cumsum(size, 0) - size).contiguous()
+        counts = torch.bincount(fd.long(), minlength=len(TILE_BYTES)).cpu().tolist()
+        cnt = {f: int(counts[f]) for f in TILE_BYTES}
+        total = sum(cnt[f] * TILE_BYTES[f] for f in TILE_BYTES)
         pool = torch.empty(max(16, total), dtype=torch.uint8, device=self.dev)
-        fd = torch.from_numpy(fmt).to(self.dev)
-        od = torch.from_numpy(off).to(self.dev)
         nat.call("qc_sm_pack", nat.ptr(M), self.nbf, nat.ptr(fd), nat.ptr(od), nat.ptr(pool),
                  nat.stream_ptr())
         # FP32 maxima rounded up, so a screen never undershoots a tile's bound
-        t32 = torch.from_numpy(np.nextafter(tm.astype(np.float32), np.float32(np.inf))).to(self.dev)
-        cnt = {f: int((fmt == f).sum()) for f in TILE_BYTES}
+        t32 = torch.nextafter(tmax.float(), torch.full_like(tmax, float("inf"), dtype=torch.float32))
         return CompactSupermatrix(pool, fd, od, t32, cnt[TILE_ZERO], cnt[TILE_F32], cnt[TILE_F64],
                                   total, tau_f32, tau_zero, cnt[TILE_I16], cnt[TILE_I8],
                                   tau_i16, tau_i8)
