@@ -11,6 +11,7 @@
 // (Raffenetti's supermatrix), so each Fock build is one symmetric
 // matrix-vector product that streams every unique integral once from HBM.
 #include <mutex>
+#include <type_traits>
 
 #include "qcmd_device.h"
 #include "qcmd_internal.h"
@@ -526,16 +527,13 @@ __global__ __launch_bounds__(256) void tile_pack_kernel(const double* __restrict
 
 // y[h][R*64 + r] = sum of the row partials of row R's items plus the column
 // partials of tiles (R', R), R' > R.  RW waves split the terms, each keeping
-// four independent sums so that four partial-block loads are in flight per
+// eight independent sums so that eight partial-block loads are in flight per
 // wave (the column partials of one row are 512-byte blocks far apart in the
-// workspace: one load at a time per wave left the reduce latency-bound).
-constexpr int RW = 8;
-
-__device__ __forceinline__ bool reduce_live(const uint8_t* __restrict__ fmt,
-                                            const TileScreen& scr, int64_t tile, int64_t Rp,
-                                            float xmR) {
-  return !scr.tmax || tile_live(fmt, scr, tile, scr.xmax[Rp], xmR);
-}
+// workspace, and row 0 has T - 1 of them: the reduce is latency-bound on its
+// first rows).  A screened tile's block is read whether or not its pass
+// wrote it (every tile has a slot) and dropped by a select, so the load does
+// not wait for the screen's own loads.
+constexpr int RW = 16;
 
 __global__ __launch_bounds__(64 * RW) void symv_reduce_kernel(int nrhs, int64_t T, int64_t npad,
                                                               const double* __restrict__ rowbuf,
@@ -546,24 +544,57 @@ __global__ __launch_bounds__(64 * RW) void symv_reduce_kernel(int nrhs, int64_t 
   const int64_t R = blockIdx.x;
   const int h = blockIdx.y;
   const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int64_t i0 = item_off(R, seg), i1 = item_off(R + 1, seg);
-  for (int64_t it = i0 + w; it < i1; it += RW) a0 += rowbuf[(it * nrhs + h) * SM_B + r];
+  for (int64_t it = i0 + w; it < i1; it += RW) a[0] += rowbuf[(it * nrhs + h) * SM_B + r];
   const float xmR = scr.tmax ? scr.xmax[R] : 0.0f;
-  auto col = [&](int64_t Rp) -> double {
+  // the tile screen (tile_live, taken at a column block), without short-circuits:
+  // every load issued
+  auto col = [&](int64_t Rp, auto screened) -> double {
     const int64_t tile = Rp * (Rp + 1) / 2 + R;
-    return reduce_live(fmt, scr, tile, Rp, xmR) ? colbuf[(tile * nrhs + h) * SM_B + r] : 0.0;
+    const double v = __builtin_nontemporal_load(colbuf + (tile * nrhs + h) * SM_B + r);
+    if constexpr (decltype(screened)::value) {
+      const bool live = (fmt[tile] != TILE_ZERO) & (scr.tmax[tile] * fmaxf(scr.xmax[Rp], xmR) >= scr.tau);
+      return live ? v : 0.0;
+    } else {
+      return v;
+    }
   };
-  int64_t Rp = R + 1 + w;
-  for (; Rp + 3 * RW < T; Rp += 4 * RW) {
-    a0 += col(Rp);
-    a1 += col(Rp + RW);
-    a2 += col(Rp + 2 * RW);
-    a3 += col(Rp + 3 * RW);
-  }
-  for (; Rp < T; Rp += RW) a0 += col(Rp);
+  auto sweep = [&](auto screened) {
+    int64_t Rp = R + 1 + w;
+    for (; Rp + 7 * RW < T; Rp += 8 * RW) {
+      // eight blocks' loads (partials, formats, maxima) first, then the sums
+      double v[8];
+      int f[8];
+      float tm[8], xm[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t Ru = Rp + u * RW, tile = Ru * (Ru + 1) / 2 + R;
+        v[u] = __builtin_nontemporal_load(colbuf + (tile * nrhs + h) * SM_B + r);
+        if constexpr (decltype(screened)::value) {
+          f[u] = fmt[tile];
+          tm[u] = scr.tmax[tile];
+          xm[u] = scr.xmax[Ru];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if constexpr (decltype(screened)::value) {
+          const bool live = (f[u] != TILE_ZERO) & (tm[u] * fmaxf(xm[u], xmR) >= scr.tau);
+          a[u] += live ? v[u] : 0.0;
+        } else {
+          a[u] += v[u];
+        }
+      }
+    }
+    for (; Rp < T; Rp += RW) a[0] += col(Rp, screened);
+  };
+  if (scr.tmax)
+    sweep(std::true_type{});
+  else
+    sweep(std::false_type{});
   __shared__ double part[RW][64];
-  part[w][r] = (a0 + a1) + (a2 + a3);
+  part[w][r] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   __syncthreads();
   if (w == 0) {
     double v = 0.0;
