@@ -41,10 +41,16 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
     }
   double gA[3] = {0, 0, 0};  // S and T parts (B = -A)
   double vA[3] = {0, 0, 0}, vB[3] = {0, 0, 0};
-  // the nuclei are split over blockIdx.y; the S and T terms belong to y == 0
+  // the nuclei are split over blockIdx.y (one per thread up to 64 atoms); the
+  // S and T terms belong to y == 0.  A nucleus's term is summed over the
+  // primitive pairs in registers and added once (the threads of a block all
+  // add to the same three components: one atomic per primitive pair made
+  // them contend)
   const bool st = blockIdx.y == 0;
 
-  for (int ip = 0; ip < np; ++ip) {
+  for (int c0 = blockIdx.y, first = 1; first || c0 < natom; c0 += gridDim.y, first = 0) {
+  double vC[3] = {0.0, 0.0, 0.0};
+  for (int ip = blockIdx.z; ip < np; ip += gridDim.z) {  // primitive pairs over z (few-pair classes)
     const double* P = pp + (int64_t)(p0 + ip) * PP_STRIDE;
     const double p = P[PP_P], K = P[PP_K], al = P[PP_ALPHA], be = p - al;
     const double oo2p = 0.5 / p;
@@ -81,7 +87,7 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
     for (int a = 0; a < NA; ++a)
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        if (!st) continue;
+        if (!st || !first) continue;
         const int ai[3] = {CX[LA][a], CY[LA][a], CZ[LA][a]};
         const int bi[3] = {CX[LB][b], CY[LB][b], CZ[LB][b]};
 #pragma unroll
@@ -109,7 +115,8 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
         }
       }
     // nuclear attraction: E up to (LA+1, LB+1) (already in Ex..), R up to L+1
-    for (int c = blockIdx.y; c < natom; c += gridDim.y) {
+    if (c0 < natom) {
+      const int c = c0;
       const double X = P[PP_PX] - axyz[3 * c], Y = P[PP_PX + 1] - axyz[3 * c + 1],
                    Z = P[PP_PX + 2] - axyz[3 * c + 2];
       double F[L + 2];
@@ -167,9 +174,13 @@ __global__ __launch_bounds__(128) void grad1e_kernel(
       for (int k = 0; k < 3; ++k) {
         vA[k] += pref * cA[k];
         vB[k] += pref * cB[k];
-        atomicAdd(&grad[3 * c + k], -pref * (cA[k] + cB[k]));
+        vC[k] -= pref * (cA[k] + cB[k]);
       }
     }
+  }
+  if (c0 < natom)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) atomicAdd(&grad[3 * c0 + k], vC[k]);
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -253,7 +264,11 @@ int qc_grad1e(int cab, const double* pairs64, const int* sp_prim_off, const int*
   hipStream_t s = (hipStream_t)stream;
   // pairs over x, nuclei over y (a pair's nuclear loop is the long serial
   // part: few pairs of the high classes, many atoms)
-  dim3 grid((npairs + 127) / 128, natom < 64 ? (natom > 0 ? natom : 1) : 64), block(128);
+  // and, for classes with few pairs, their primitive pairs over z ((H2O)_16:
+  // the 136 (dd) pairs filled 2 x 48 blocks of 128 threads, a tenth of the
+  // SIMDs, each thread a spilling serial loop over nine primitive pairs)
+  const int pz = npairs < 1024 ? 9 : 1;
+  dim3 grid((npairs + 127) / 128, natom < 64 ? (natom > 0 ? natom : 1) : 64, pz), block(128);
 #define QC_G1(A, B)                                                                             \
   case (A) * ((A) + 1) / 2 + (B):                                                               \
     hipLaunchKernelGGL((grad1e_kernel<A, B>), grid, block, 0, s, pairs64, sp_prim_off, sp_nprim, \
