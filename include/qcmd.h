@@ -88,6 +88,11 @@ int qc_one_electron(int which, int cab, const double* pairs64, const int* sp_pri
  * output are tile-major packed supermatrices over nbf functions. */
 int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* stream);
 
+/* J <- J - K/2 in place over two tile-major supermatrices of nbf functions:
+ * turns J into the P of kind 0 above once K (kind 1) is assembled, with no
+ * third supermatrix.  J and K must be distinct. */
+int qc_sm_p_from_jk(double* J, const double* K, int64_t nbf, void* stream);
+
 /* Launch shape of the Fock-build tile pass: tiles per work item (1..64) and
  * whether tile loads are non-temporal.  Process-wide. */
 int qc_symv_config(int seg, int nontemporal);

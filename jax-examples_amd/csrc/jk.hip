@@ -76,31 +76,72 @@ __device__ inline int64_t pidx(int64_t i, int64_t j) {
   return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }
 
-__global__ void sm_assemble_kernel(int kind, const double* __restrict__ J, double* __restrict__ out,
-                                   int64_t nbf, int64_t nelem) {
-  const int64_t e = flat_block() * blockDim.x + threadIdx.x;
-  if (e >= nelem) return;
+// One work-group per 64 x 64 output tile (R, C); lane c of wave w computes
+// the elements (r, c), r = w, w + 4, ..., w + 60, so each wave writes whole
+// tile rows.  Row r is the pair p = (ij), column c the pair q = (kl), and
+// K_pq = [(ik|jl) + (il|jk)] / 2 (both orders of p and q give the same
+// value, so diagonal tiles hold both triangles).  Over the tile the first
+// integral reads row (ik) of J at the 64 x 64 columns (jl), and the second
+// the 64 x 64 block of rows (il), columns (jk), both gathered element by
+// element; where j > k the second has no contiguous direction in the tile
+// (one cache line per element), which bounds the kernel: (H2O)_16, 53 GB
+// per matrix, 63 ms (68 ms with a work-group per four tile rows).
+__global__ __launch_bounds__(256) void sm_assemble_kernel(int kind, const double* __restrict__ J,
+                                                          double* __restrict__ out, int64_t nbf,
+                                                          int64_t ntiles) {
+  const int64_t t = flat_block();
+  if (t >= ntiles) return;
   const int64_t npair = nbf * (nbf + 1) / 2;
-  const int64_t t = e >> (2 * SM_LOGB);
-  const int64_t r = (e >> SM_LOGB) & (SM_B - 1), c = e & (SM_B - 1);
   const int64_t R = tri_root(t), C = t - R * (R + 1) / 2;
-  int64_t p = R * SM_B + r, q = C * SM_B + c;
-  if (p >= npair || q >= npair) {
-    out[e] = 0.0;
-    return;
-  }
-  if (p < q) {
-    int64_t tmp = p;
-    p = q;
-    q = tmp;
-  }
-  const int64_t i = tri_root(p), j = p - i * (i + 1) / 2;
+  const int c = threadIdx.x & (SM_B - 1), w = threadIdx.x >> SM_LOGB;
+  const int64_t q = C * SM_B + c;
+  const bool qok = q < npair;
   const int64_t k = tri_root(q), l = q - k * (k + 1) / 2;
-  const double jv = J[sm_pos2(p, q)];
-  const double v1 = J[sm_pos2(pidx(i, k), pidx(j, l))];
-  const double v2 = J[sm_pos2(pidx(i, l), pidx(j, k))];
-  const double kv = 0.5 * (v1 + v2);
-  out[e] = kind == 0 ? jv - 0.5 * kv : kv;
+  int64_t p = R * SM_B + w;
+  int64_t i = tri_root(p), j = p - i * (i + 1) / 2;
+  double* o = out + (t << (2 * SM_LOGB)) + c;
+  for (int r = w; r < SM_B; r += 4) {
+    double v = 0.0;
+    if (qok && p < npair) {
+      const double v1 = J[sm_pos2(pidx(i, k), pidx(j, l))];
+      const double v2 = J[sm_pos2(pidx(i, l), pidx(j, k))];
+      const double kv = 0.5 * (v1 + v2);
+      v = kind == 0 ? J[sm_pos2(p, q)] - 0.5 * kv : kv;
+    }
+    __builtin_nontemporal_store(v, o + r * SM_B);
+    p += 4;  // next row of this wave: (ij) four pairs on
+    j += 4;
+    while (j > i) {
+      j -= i + 1;
+      ++i;
+    }
+  }
+}
+
+// J <- J - K/2 over n2 pairs of doubles (every element of both tile-major
+// supermatrices, padding included, which is zero in both).  A work-group
+// takes 1024 pairs, four per lane, all eight loads issued before the
+// stores (one pair per lane in a grid-stride loop ran at 5.0 TB/s).
+__global__ __launch_bounds__(256) void sm_p_from_jk_kernel(double* __restrict__ J,
+                                                           const double* __restrict__ K,
+                                                           int64_t n2) {
+  const int64_t e0 = flat_block() * 1024 + threadIdx.x;
+  dvec2* j2 = reinterpret_cast<dvec2*>(J);
+  const dvec2* k2 = reinterpret_cast<const dvec2*>(K);
+  dvec2 a[4], b[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t e = e0 + 256 * u;
+    if (e < n2) {
+      a[u] = __builtin_nontemporal_load(j2 + e);
+      b[u] = __builtin_nontemporal_load(k2 + e);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t e = e0 + 256 * u;
+    if (e < n2) __builtin_nontemporal_store(a[u] - 0.5 * b[u], j2 + e);
+  }
 }
 
 // Work item m of the SYMV: tile row R, segment s covering tiles
@@ -753,10 +794,20 @@ int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* 
   if (nbf <= 0) return set_error("qc_sm_assemble: nbf %lld", (long long)nbf);
   if (Jsm == out) return set_error("qc_sm_assemble: in-place assembly is not supported");
   const int64_t T = sm_tiles_per_dim(nbf);
-  const int64_t nelem = (T * (T + 1) / 2) << (2 * SM_LOGB);
-  hipLaunchKernelGGL(sm_assemble_kernel, grid_1d((nelem + 255) / 256, 256), dim3(256), 0,
-                     (hipStream_t)stream, kind, Jsm, out, nbf, nelem);
+  const int64_t ntiles = T * (T + 1) / 2;
+  hipLaunchKernelGGL(sm_assemble_kernel, grid_1d(ntiles, 256), dim3(256), 0, (hipStream_t)stream,
+                     kind, Jsm, out, nbf, ntiles);
   return check(hipGetLastError(), "qc_sm_assemble launch");
+}
+
+int qc_sm_p_from_jk(double* J, const double* K, int64_t nbf, void* stream) {
+  if (nbf <= 0) return set_error("qc_sm_p_from_jk: nbf %lld", (long long)nbf);
+  if (!J || !K || J == K) return set_error("qc_sm_p_from_jk: J and K must be distinct buffers");
+  const int64_t T = sm_tiles_per_dim(nbf);
+  const int64_t n2 = ((T * (T + 1) / 2) << (2 * SM_LOGB)) / 2;
+  hipLaunchKernelGGL(sm_p_from_jk_kernel, grid_1d((n2 + 1023) / 1024, 256), dim3(256), 0,
+                     (hipStream_t)stream, J, K, n2);
+  return check(hipGetLastError(), "qc_sm_p_from_jk launch");
 }
 
 int qc_symv_config(int seg, int nontemporal) {

@@ -152,7 +152,7 @@ class ERIStore:
         K = self.ensure_K()
         if self.P is None:
             self._assemble_from_J("P")
-            self.J.add_(K, alpha=-0.5)
+            nat.call("qc_sm_p_from_jk", nat.ptr(self.J), nat.ptr(K), self.nbf, nat.stream_ptr())
             self.P, self.J = self.J, None
         return self.P, K
 
