@@ -340,32 +340,24 @@ __device__ __forceinline__ bool tile_live(const uint8_t* __restrict__ fmt, const
   return scr.tmax[tile] * fmaxf(xa, xb) >= scr.tau;
 }
 
-// One wave per work item (tile row R, a segment of its tiles) of one
-// supermatrix whose tiles start at global tile tile0 and whose packed vector
-// rows start at global row row0 (0 and 0 for a single matrix; many
-// molecules share one launch otherwise).  Row partials accumulate in
-// registers over the segment and are reduced once at the end by recursive
-// halving over the 16 lanes sharing ro.  MIXED: tiles are read through a
-// format/offset directory (zero tiles skipped, FP32 or FP64, FP64
-// accumulation either way).
-template <int NRHS, bool MIXED, bool NT, int WS>
-__device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T,
-                                          const uint8_t* __restrict__ fmt,
-                                          const int64_t* __restrict__ off,
-                                          const double* __restrict__ x, int64_t npad,
-                                          double* __restrict__ rowbuf, double* __restrict__ colbuf,
-                                          int seg, int64_t item, int64_t gitem, int64_t tile0,
-                                          int64_t row0, const TileScreen& scr) {
-  int64_t lo = 0, hi = T - 1;
-  while (lo < hi) {
-    int64_t mid = (lo + hi + 1) >> 1;
-    if (item_off(mid, seg) <= item)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const int64_t R = lo, s = item - item_off(R, seg);
-  const int64_t C0 = s * seg, C1 = min(C0 + seg, R + 1);
+// Tiles (R, C0 .. C1-1) of one supermatrix whose tiles start at global tile
+// tile0 and whose packed vector rows start at global row row0 (0 and 0 for a
+// single matrix; many molecules share one launch otherwise).  Row partials
+// accumulate in registers over the segment and are reduced once at the end
+// by recursive halving over the 16 lanes sharing ro, into rowbuf block
+// gitem.  Column partials of tile (R, C) go to colbuf block `tile` (the
+// global tile index) or, GROUPED, to the wave's LDS slots: colbuf then
+// points at them and the slot is C - C0.  MIXED: tiles are read through a
+// format/offset directory (zero tiles skipped, FP32 / FP64 / int16 / int8,
+// FP64 accumulation either way).
+template <int NRHS, bool MIXED, bool NT, int WS, bool GROUPED = false>
+__device__ __forceinline__ void symv_segment(const void* __restrict__ Mv,
+                                             const uint8_t* __restrict__ fmt,
+                                             const int64_t* __restrict__ off,
+                                             const double* __restrict__ x, int64_t npad,
+                                             double* __restrict__ rowbuf, double* colbuf,
+                                             int64_t R, int64_t C0, int64_t C1, int64_t gitem,
+                                             int64_t tile0, int64_t row0, const TileScreen& scr) {
   constexpr int ROWS = SYMV_ROWS / WS;
   const int lane = threadIdx.x & 63, cg = lane & 15, ro = lane >> 4;
   const int wave = WS == 1 ? 0 : (int)(threadIdx.x >> 6);
@@ -398,11 +390,12 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
 
   for (int64_t C = C0; C < C1; ++C) {
     const int64_t tile = tile0 + R * (R + 1) / 2 + C;
+    const int64_t dtile = GROUPED ? C - C0 : tile;  // column-partial block
     const bool diag = C == R;
     const double* xcb = x + (row0 + C) * SM_B;
     if (!MIXED) {
       const double* tp = reinterpret_cast<const double*>(Mv) + (tile << (2 * SM_LOGB));
-      symv_tile<NRHS, WS, NT, double>(tp, tile, diag, xr, rowacc, xcb, npad, cg, ro, wave, par,
+      symv_tile<NRHS, WS, NT, double>(tp, dtile, diag, xr, rowacc, xcb, npad, cg, ro, wave, par,
                                       sh, colbuf);
     } else {
       const int c = (int)(C - C0);
@@ -412,19 +405,19 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
       const uint32_t ohi = __builtin_amdgcn_readlane((uint32_t)(dir_o >> 32), c);
       const char* base = reinterpret_cast<const char*>(Mv) + (int64_t)(((uint64_t)ohi << 32) | olo);
       if (f == TILE_F64) {
-        symv_tile<NRHS, WS, NT, double>(reinterpret_cast<const double*>(base), tile, diag, xr,
+        symv_tile<NRHS, WS, NT, double>(reinterpret_cast<const double*>(base), dtile, diag, xr,
                                         rowacc, xcb, npad, cg, ro, wave, par, sh, colbuf);
       } else if (f == TILE_F32) {
-        symv_tile<NRHS, WS, NT, float>(reinterpret_cast<const float*>(base), tile, diag, xr,
+        symv_tile<NRHS, WS, NT, float>(reinterpret_cast<const float*>(base), dtile, diag, xr,
                                        rowacc, xcb, npad, cg, ro, wave, par, sh, colbuf);
       } else if (f == TILE_I16) {
         const double sc = *reinterpret_cast<const double*>(base);  // uniform: one scalar load
-        symv_tile<NRHS, WS, NT, int16_t>(reinterpret_cast<const int16_t*>(base + TILE_QHDR), tile,
+        symv_tile<NRHS, WS, NT, int16_t>(reinterpret_cast<const int16_t*>(base + TILE_QHDR), dtile,
                                          diag, xr, rowacc, xcb, npad, cg, ro, wave, par, sh,
                                          colbuf, sc);
       } else if (f == TILE_I8) {
         const double sc = *reinterpret_cast<const double*>(base);
-        symv_tile<NRHS, WS, NT, int8_t>(reinterpret_cast<const int8_t*>(base + TILE_QHDR), tile,
+        symv_tile<NRHS, WS, NT, int8_t>(reinterpret_cast<const int8_t*>(base + TILE_QHDR), dtile,
                                         diag, xr, rowacc, xcb, npad, cg, ro, wave, par, sh, colbuf,
                                         sc);
       } else if (!diag && ro == 0 && wave == 0) {
@@ -432,7 +425,7 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
         for (int h = 0; h < NRHS; ++h) {
           dvec2 z;
           z.x = z.y = 0.0;
-          dvec2* dst = reinterpret_cast<dvec2*>(colbuf + (tile * NRHS + h) * SM_B + 4 * cg);
+          dvec2* dst = reinterpret_cast<dvec2*>(colbuf + (dtile * NRHS + h) * SM_B + 4 * cg);
           dst[0] = z;
           dst[1] = z;
         }
@@ -465,6 +458,30 @@ __device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T
   }
 }
 
+// One wave (WS = 1) or two (WS = 2, each half of every tile's rows) per
+// work item: tile row R, a segment of its tiles (see item_off).
+template <int NRHS, bool MIXED, bool NT, int WS>
+__device__ __forceinline__ void symv_item(const void* __restrict__ Mv, int64_t T,
+                                          const uint8_t* __restrict__ fmt,
+                                          const int64_t* __restrict__ off,
+                                          const double* __restrict__ x, int64_t npad,
+                                          double* __restrict__ rowbuf, double* __restrict__ colbuf,
+                                          int seg, int64_t item, int64_t gitem, int64_t tile0,
+                                          int64_t row0, const TileScreen& scr) {
+  int64_t lo = 0, hi = T - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (item_off(mid, seg) <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int64_t R = lo, s = item - item_off(R, seg);
+  const int64_t C0 = s * seg, C1 = min(C0 + seg, R + 1);
+  symv_segment<NRHS, MIXED, NT, WS>(Mv, fmt, off, x, npad, rowbuf, colbuf, R, C0, C1, gitem, tile0,
+                                    row0, scr);
+}
+
 template <int NRHS, bool MIXED, bool NT, int WS>
 __global__ __launch_bounds__(64 * WS, SYMV_MIN_WAVES) void symv_tiles_kernel(
     const void* __restrict__ Mv, int64_t T, const uint8_t* __restrict__ fmt,
@@ -473,6 +490,77 @@ __global__ __launch_bounds__(64 * WS, SYMV_MIN_WAVES) void symv_tiles_kernel(
   const int64_t item = blockIdx.x;
   symv_item<NRHS, MIXED, NT, WS>(Mv, T, fmt, off, x, npad, rowbuf, colbuf, seg, item, item, 0, 0,
                                  scr);
+}
+
+// Grouped tile pass for one right-hand side: a work-group of SYMV_G = 4
+// waves takes tile rows R = 4g + w (wave w) over one column segment
+// [seg s, seg s + seg) (seg a multiple of 4, at most SYMV_GSEG).  Each wave
+// writes its tiles' column partials into its own LDS slots (zeros where it
+// has none), and after one barrier the work-group sums the four slots of
+// each column in a fixed order and writes one 512-byte block per (row group
+// g, column C) instead of one per tile: the tile pass writes and the reduce
+// reads a quarter of the column partials (0.83 GB per pass over the
+// 1,628,110 tiles of a 480-function supermatrix).  Block (g, C) is the sum
+// over the group's rows R' > C (tile (C, C) is a diagonal tile, whose column
+// half the row sums already hold, and tiles (R', C) with R' < C do not
+// exist); every block of the group's columns 0 .. min(4g+3, T-1) is
+// written, zeros included, so the reduce needs no screen.  Group g has
+// g / (seg/4) + 1 segments (its last row's), so item_off(g, seg/4) counts
+// the work-groups before it.
+constexpr int SYMV_G = 4;
+constexpr int SYMV_GSEG = 16;
+
+__host__ __device__ inline int64_t gcol_off(int64_t g) {  // first column block of group g
+  return 2 * g * (g + 1);                                 // 4g' + 4 for each group g' < g
+}
+__host__ __device__ inline int64_t symv_group_blocks(int64_t T) {
+  const int64_t NG = (T + SYMV_G - 1) / SYMV_G;
+  return gcol_off(NG - 1) + T;  // the last group's columns 0 .. T-1
+}
+static bool symv_grouped(int nrhs, int seg, int ws) {
+  return nrhs == 1 && ws == 1 && seg % SYMV_G == 0 && seg <= SYMV_GSEG;
+}
+// Row and column partials of the grouped layout, in 64-double blocks.
+static void symv_group_layout(int64_t T, int seg, int64_t* items, int64_t* blocks) {
+  const int64_t NG = (T + SYMV_G - 1) / SYMV_G;
+  *items = item_off(NG, seg / SYMV_G);
+  *blocks = *items * SYMV_G + symv_group_blocks(T);
+}
+
+template <bool MIXED, bool NT>
+__global__ __launch_bounds__(64 * SYMV_G) void symv_group_kernel(
+    const void* __restrict__ Mv, int64_t T, const uint8_t* __restrict__ fmt,
+    const int64_t* __restrict__ off, const double* __restrict__ x, int64_t npad,
+    double* __restrict__ rowbuf, double* __restrict__ colbuf, int seg, TileScreen scr) {
+  __shared__ double slot[SYMV_G][SYMV_GSEG][SM_B];
+  const int64_t item = blockIdx.x;
+  const int sg = seg / SYMV_G;
+  const int64_t NG = (T + SYMV_G - 1) / SYMV_G;
+  int64_t lo = 0, hi = NG - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (item_off(mid, sg) <= item)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int64_t g = lo, C0 = (item - item_off(g, sg)) * seg;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t R = g * SYMV_G + w;
+  double* mine = &slot[w][0][0];
+#pragma unroll
+  for (int k = 0; k < SYMV_GSEG; ++k) mine[k * SM_B + lane] = 0.0;
+  if (R < T)  // C0 <= R: every row of the group reaches every segment of it
+    symv_segment<1, MIXED, NT, 1, true>(Mv, fmt, off, x, npad, rowbuf, mine, R, C0,
+                                        min(C0 + seg, R + 1), item * SYMV_G + w, 0, 0, scr);
+  __syncthreads();
+  const int64_t Rmax = min(g * SYMV_G + SYMV_G - 1, T - 1);
+  const int ncol = (int)min((int64_t)seg, Rmax + 1 - C0);
+  double* dst = colbuf + (gcol_off(g) + C0) * SM_B;
+  for (int e = threadIdx.x; e < ncol * SM_B; e += 64 * SYMV_G) {
+    const int c = e >> SM_LOGB, r = e & (SM_B - 1);
+    dst[e] = ((slot[0][c][r] + slot[1][c][r]) + slot[2][c][r]) + slot[3][c][r];
+  }
 }
 
 // Last index i in [0, n) with pre[i] <= v (pre ascending, pre[0] = 0).
@@ -642,6 +730,41 @@ __global__ __launch_bounds__(64 * RW) void symv_reduce_kernel(int nrhs, int64_t 
 #pragma unroll
     for (int k = 0; k < RW; ++k) v += part[k][r];
     y[h * npad + R * SM_B + r] = v;
+  }
+}
+
+// Reduce of the grouped layout: y[R] = the row partials of row R's
+// work-groups plus the column blocks (g', R) of the groups g' >= R / 4.
+__global__ __launch_bounds__(64 * RW) void symv_group_reduce_kernel(int64_t T, int64_t npad,
+                                                                    const double* __restrict__ rowbuf,
+                                                                    const double* __restrict__ colbuf,
+                                                                    double* __restrict__ y, int seg) {
+  const int64_t R = blockIdx.x;
+  const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int sg = seg / SYMV_G;
+  const int64_t g = R / SYMV_G, NG = (T + SYMV_G - 1) / SYMV_G;
+  const int wr = (int)(R - g * SYMV_G);
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int64_t i0 = item_off(g, sg), i1 = item_off(g + 1, sg);
+  for (int64_t it = i0 + w; it < i1; it += RW) a[0] += rowbuf[(it * SYMV_G + wr) * SM_B + r];
+  int64_t gp = g + w;
+  for (; gp + 7 * RW < NG; gp += 8 * RW) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      v[u] = __builtin_nontemporal_load(colbuf + (gcol_off(gp + u * RW) + R) * SM_B + r);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += v[u];
+  }
+  for (; gp < NG; gp += RW) a[1] += colbuf[(gcol_off(gp) + R) * SM_B + r];
+  __shared__ double part[RW][64];
+  part[w][r] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  __syncthreads();
+  if (w == 0) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < RW; ++k) v += part[k][r];
+    y[R * SM_B + r] = v;
   }
 }
 
@@ -825,9 +948,29 @@ int qc_symv_split(int nrhs, int waves) {
   return 0;
 }
 
+// Workspace of the layout a call with this configuration uses.
+static int64_t symv_need_bytes(int64_t nbf, int nrhs) {
+  if (symv_grouped(nrhs, g_symv_seg, g_symv_ws[1])) {
+    int64_t items, blocks;
+    symv_group_layout(sm_tiles_per_dim(nbf), g_symv_seg, &items, &blocks);
+    return blocks * SM_B * (int64_t)sizeof(double);
+  }
+  return symv_work_bytes(nbf, nrhs, g_symv_seg);
+}
+
 int64_t qc_symv_workspace_bytes(int64_t nbf, int nrhs) {
-  // sized for the smallest supported segment so any later qc_symv_config fits
-  return symv_work_bytes(nbf, nrhs, 1);
+  // sized for any later qc_symv_config / qc_symv_split: the per-tile layout
+  // at segment 1 and, for one right-hand side, the grouped layout at its
+  // smallest segment (4), which needs more for a single tile row (4 row
+  // blocks per work-group)
+  int64_t need = symv_work_bytes(nbf, nrhs, 1);
+  if (nrhs == 1) {
+    int64_t items, blocks;
+    symv_group_layout(sm_tiles_per_dim(nbf), SYMV_G, &items, &blocks);
+    const int64_t g = blocks * SM_B * (int64_t)sizeof(double);
+    if (g > need) need = g;
+  }
+  return need;
 }
 
 static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const double* x,
@@ -839,11 +982,33 @@ static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const d
   if (!M || !x || !work || (phase != 1 && !y)) return set_error("qc_symv: null pointer");
   const int64_t T = sm_tiles_per_dim(nbf), npad = T * SM_B;
   const int seg = g_symv_seg;
-  const int64_t items = item_off(T, seg);
+  const bool grouped = symv_grouped(nrhs, seg, g_symv_ws[1]);
+  int64_t items = item_off(T, seg), gblocks = 0;
+  if (grouped) symv_group_layout(T, seg, &items, &gblocks);
   double* rowbuf = (double*)work;
-  double* colbuf = rowbuf + items * nrhs * SM_B;
+  double* colbuf = rowbuf + items * (grouped ? SYMV_G : nrhs) * SM_B;
   hipStream_t s = (hipStream_t)stream;
-  if (phase & 1) {
+  if ((phase & 1) && grouped) {
+    if (fmt) {
+      if (g_symv_nt)
+        hipLaunchKernelGGL((symv_group_kernel<true, true>), dim3((unsigned)items), dim3(64 * SYMV_G),
+                           0, s, (const void*)M, T, fmt, off, x, npad, rowbuf, colbuf, seg, scr);
+      else
+        hipLaunchKernelGGL((symv_group_kernel<true, false>), dim3((unsigned)items),
+                           dim3(64 * SYMV_G), 0, s, (const void*)M, T, fmt, off, x, npad, rowbuf,
+                           colbuf, seg, scr);
+    } else {
+      if (g_symv_nt)
+        hipLaunchKernelGGL((symv_group_kernel<false, true>), dim3((unsigned)items),
+                           dim3(64 * SYMV_G), 0, s, (const void*)M, T, fmt, off, x, npad, rowbuf,
+                           colbuf, seg, scr);
+      else
+        hipLaunchKernelGGL((symv_group_kernel<false, false>), dim3((unsigned)items),
+                           dim3(64 * SYMV_G), 0, s, (const void*)M, T, fmt, off, x, npad, rowbuf,
+                           colbuf, seg, scr);
+    }
+    if (check(hipGetLastError(), "qc_symv tiles")) return 1;
+  } else if (phase & 1) {
     switch (nrhs) {
 #define QC_L(N, MX, NT)                                                                        \
   do {                                                                                         \
@@ -874,7 +1039,11 @@ static int symv_phase(int phase, const double* M, int64_t nbf, int nrhs, const d
     }
     if (check(hipGetLastError(), "qc_symv tiles")) return 1;
   }
-  if (phase & 2) {
+  if ((phase & 2) && grouped) {
+    hipLaunchKernelGGL(symv_group_reduce_kernel, dim3((unsigned)T), dim3(64 * RW), 0, s, T, npad,
+                       rowbuf, colbuf, y, seg);
+    if (check(hipGetLastError(), "qc_symv reduce")) return 1;
+  } else if (phase & 2) {
     hipLaunchKernelGGL(symv_reduce_kernel, dim3((unsigned)T, nrhs), dim3(64 * RW), 0, s, nrhs, T,
                        npad, rowbuf, colbuf, y, seg, fmt, scr);
     if (check(hipGetLastError(), "qc_symv reduce")) return 1;
@@ -891,7 +1060,7 @@ int qc_symv_noalloc(const double* M, int64_t nbf, int nrhs, const double* x, dou
                     void* stream) {
   if (nrhs < 1 || nrhs > 4) return set_error("qc_symv_noalloc: nrhs %d outside [1, 4]", nrhs);
   if (nbf <= 0) return set_error("qc_symv_noalloc: nbf %lld", (long long)nbf);
-  const int64_t need = symv_work_bytes(nbf, nrhs, g_symv_seg);
+  const int64_t need = symv_need_bytes(nbf, nrhs);
   if (need > kSymvScratchBytes)
     return set_error("qc_symv_noalloc: nbf %lld with %d right-hand sides needs %lld B of "
                      "workspace, above the %lld B scratch (use qc_symv with a caller workspace)",
