@@ -24,8 +24,9 @@ without the incremental shortcut.  ``value`` = mixed Fock
 builds/s; the config carries both paths' Fock rates, energy + gradient wall
 times, the speed-up, |dE| and max |dgrad|.  The roofline is that of the
 dominant kernel of the mixed energy + gradient (the P tile pass,
-symv_tiles_kernel<1,true,*>, HBM-bound; algorithmic bytes = the compacted P
-pool per launch), timed with HIP events on the launch stream.
+symv_group_kernel<true,*>, HBM-bound; algorithmic bytes = the compacted P
+pool per launch), timed with HIP events on the launch stream, beside the
+read rate torch's sum reaches over as many bytes (a practical ceiling).
 cpu_baseline = the oracle's UHF Fock build (three np.einsum contractions of
 the reference's jax_qc/scf/fock.py:31-79) on a slab of a dense tensor,
 scaled to one build; the same line carries an extrapolated CPU ERI build
@@ -56,19 +57,20 @@ from jax_qc.precision import MIXED, MIXED_GRAD, STRICT  # noqa: E402
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
-# HBM bytes per launch of the Fock-build tile kernel on the (H2O)_8 workload,
-# from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script, each
-# scaled by the same counters on a 1 GiB device copy (FETCH x2.00, WRITE
-# x1.00); profiles/symv_tiles_water8_*.md hold the commands and counts.
-MEASURED_TRAFFIC = {"fp64": (1639021.6 * 2.0 + 54269.0) * 1024,
-                    "mixed": (1152157.2 * 2.0 + 63845.3) * 1024}
-# the same for the P tile pass (symv_tiles_kernel<1,true,true,1> over the
+# HBM bytes per launch of the Fock-build tile kernel (symv_group_kernel) on
+# the (H2O)_8 workload, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+# this script, each scaled by the same counters on a 1 GiB device copy
+# (FETCH x2.00, WRITE x1.00); profiles/symv_tiles_water8_*.md hold the
+# commands and counts.
+MEASURED_TRAFFIC = {"fp64": (1639096.8 * 2.0 + 16188.0) * 1024,
+                    "mixed": (1150726.7 * 2.0 + 16188.0) * 1024}
+# the same for the P tile pass (symv_group_kernel<true,true> over the
 # compacted P = J - K/2 pool) of the (H2O)_16 mixed workload (int8 / int16 /
-# FP32 / FP64 tiles): FETCH 11,505,185 KiB x 2.00 + WRITE 856,173 KiB per
+# FP32 / FP64 tiles): FETCH 11,491,464 KiB x 2.00 + WRITE 255,154 KiB per
 # launch (mean of 5 launches, profiles/fock_and_energy_gradient_water16_uhf.md),
-# 1.06 x the 23.0 GB algorithmic pool bytes (the rest: column partials, x and
-# the tile directory)
-TRAFFIC_C5_P = (11505185 * 2 + 856173) * 1024
+# 1.034 x the 23.0 GB algorithmic pool bytes (the rest: row and column
+# partials, x and the tile directory)
+TRAFFIC_C5_P = (11491464 * 2 + 255154) * 1024
 
 
 def _cpu_info():
@@ -763,9 +765,12 @@ def run_water16(args):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": "symv_tiles_kernel<1,true,*> (P pass, compacted pool)",
+                     "kernel": "symv_group_kernel<true,*> (P pass, compacted pool)",
                      "bytes_per_launch": m["nbytes"]["P"], "avg_launch_ms": round(pms, 5)},
     }
+    ceiling = read_ceiling(m["nbytes"]["P"])
+    out["roofline"]["read_ceiling_gbs"] = round(ceiling, 1)
+    out["roofline"]["frac_of_read_ceiling"] = round(achieved / ceiling, 4)
     if TRAFFIC_C5_P is not None:
         out["roofline"]["traffic"] = TRAFFIC_C5_P
         out["roofline"]["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated"
@@ -933,7 +938,7 @@ def main():
                                  if grid == (2, 2, 2) and (pick == "fp64" or args.tile_f32 == 1e-2)
                                  else None),
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated",
-                     "kernel": "symv_tiles_kernel<1,%s>" % ("true" if pick == "mixed" else "false"),
+                     "kernel": "symv_group_kernel<%s,*>" % ("true" if pick == "mixed" else "false"),
                      "bytes_per_launch": P["bytes"], "avg_launch_ms": round(P["avg_ms"], 5),
                      "read_ceiling_gbs": round(ceiling, 1),
                      "frac_of_read_ceiling": round(achieved / ceiling, 4)},
