@@ -88,10 +88,17 @@ int qc_one_electron(int which, int cab, const double* pairs64, const int* sp_pri
  * output are tile-major packed supermatrices over nbf functions. */
 int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* stream);
 
+/* qc_sm_assemble that also writes each tile's max |element| of `out` to
+ * tmax[T(T+1)/2] (T = tiles per dimension): the tile maxima a compaction
+ * needs (qc_sm_tile_maxabs), without another read of the supermatrix. */
+int qc_sm_assemble_maxabs(int kind, const double* Jsm, double* out, int64_t nbf, double* tmax,
+                          void* stream);
+
 /* J <- J - K/2 in place over two tile-major supermatrices of nbf functions:
  * turns J into the P of kind 0 above once K (kind 1) is assembled, with no
- * third supermatrix.  J and K must be distinct. */
-int qc_sm_p_from_jk(double* J, const double* K, int64_t nbf, void* stream);
+ * third supermatrix.  J and K must be distinct.  tmax (may be NULL): each
+ * tile's max |element| of the result, as qc_sm_tile_maxabs would give. */
+int qc_sm_p_from_jk(double* J, const double* K, int64_t nbf, double* tmax, void* stream);
 
 /* Launch shape of the Fock-build tile pass: tiles per work item (1..64) and
  * whether tile loads are non-temporal.  Process-wide. */
@@ -115,9 +122,9 @@ int qc_symv(const double* M, int64_t nbf, int nrhs, const double* x, double* y, 
  * its address taken at qc_init, held until the process ends).  No
  * counterpart in the reference; it is for callers that cannot allocate
  * (code under HIP graph capture, tight serving loops).  Fails with a clear
- * error when the workspace would exceed the array (qc_symv_workspace_bytes
- * at the configured segment length; e.g. nbf <= 250 at nrhs = 1 and the
- * default segment) or when qc_init was not called on the current device.
+ * error when the workspace of the configured tile-pass layout would exceed
+ * the array (e.g. nbf <= 341 at nrhs = 1 and the default segment, 211 at
+ * nrhs = 2) or when qc_init was not called on the current device.
  * Calls that overlap on one GPU share the array and race. */
 int qc_symv_noalloc(const double* M, int64_t nbf, int nrhs, const double* x, double* y,
                     void* stream);

@@ -76,6 +76,16 @@ __device__ inline int64_t pidx(int64_t i, int64_t j) {
   return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }
 
+// Max of m over a 256-lane work-group, written to *dst by lane 0 (every
+// lane of the work-group must call it).
+__device__ __forceinline__ void tile_block_max(double m, double* dst) {
+  __shared__ double wmax[4];
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *dst = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+}
+
 // One work-group per 64 x 64 output tile (R, C); lane c of wave w computes
 // the elements (r, c), r = w, w + 4, ..., w + 60, so each wave writes whole
 // tile rows.  Row r is the pair p = (ij), column c the pair q = (kl), and
@@ -88,7 +98,7 @@ __device__ inline int64_t pidx(int64_t i, int64_t j) {
 // per matrix, 63 ms (68 ms with a work-group per four tile rows).
 __global__ __launch_bounds__(256) void sm_assemble_kernel(int kind, const double* __restrict__ J,
                                                           double* __restrict__ out, int64_t nbf,
-                                                          int64_t ntiles) {
+                                                          int64_t ntiles, double* __restrict__ tmax) {
   const int64_t t = flat_block();
   if (t >= ntiles) return;
   const int64_t npair = nbf * (nbf + 1) / 2;
@@ -100,6 +110,7 @@ __global__ __launch_bounds__(256) void sm_assemble_kernel(int kind, const double
   int64_t p = R * SM_B + w;
   int64_t i = tri_root(p), j = p - i * (i + 1) / 2;
   double* o = out + (t << (2 * SM_LOGB)) + c;
+  double m = 0.0;
   for (int r = w; r < SM_B; r += 4) {
     double v = 0.0;
     if (qok && p < npair) {
@@ -109,6 +120,7 @@ __global__ __launch_bounds__(256) void sm_assemble_kernel(int kind, const double
       v = kind == 0 ? J[sm_pos2(p, q)] - 0.5 * kv : kv;
     }
     __builtin_nontemporal_store(v, o + r * SM_B);
+    m = fmax(m, fabs(v));
     p += 4;  // next row of this wave: (ij) four pairs on
     j += 4;
     while (j > i) {
@@ -116,32 +128,36 @@ __global__ __launch_bounds__(256) void sm_assemble_kernel(int kind, const double
       ++i;
     }
   }
+  if (tmax) tile_block_max(m, tmax + t);
 }
 
-// J <- J - K/2 over n2 pairs of doubles (every element of both tile-major
-// supermatrices, padding included, which is zero in both).  A work-group
-// takes 1024 pairs, four per lane, all eight loads issued before the
-// stores (one pair per lane in a grid-stride loop ran at 5.0 TB/s).
+// J <- J - K/2 in place, one work-group per 64 x 64 tile (both tile-major
+// supermatrices; the padding is zero in both): a lane takes eight pairs of
+// doubles, all sixteen loads issued before the stores, and with tmax the
+// tile's max |J - K/2| is written to tmax[tile] (the compaction's tile
+// maxima without another read of the matrix).
 __global__ __launch_bounds__(256) void sm_p_from_jk_kernel(double* __restrict__ J,
                                                            const double* __restrict__ K,
-                                                           int64_t n2) {
-  const int64_t e0 = flat_block() * 1024 + threadIdx.x;
-  dvec2* j2 = reinterpret_cast<dvec2*>(J);
-  const dvec2* k2 = reinterpret_cast<const dvec2*>(K);
-  dvec2 a[4], b[4];
+                                                           int64_t ntiles,
+                                                           double* __restrict__ tmax) {
+  const int64_t t = flat_block();
+  if (t >= ntiles) return;
+  dvec2* j2 = reinterpret_cast<dvec2*>(J) + (t << (2 * SM_LOGB - 1)) + threadIdx.x;
+  const dvec2* k2 = reinterpret_cast<const dvec2*>(K) + (t << (2 * SM_LOGB - 1)) + threadIdx.x;
+  dvec2 a[8], b[8];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int64_t e = e0 + 256 * u;
-    if (e < n2) {
-      a[u] = __builtin_nontemporal_load(j2 + e);
-      b[u] = __builtin_nontemporal_load(k2 + e);
-    }
+  for (int u = 0; u < 8; ++u) {
+    a[u] = __builtin_nontemporal_load(j2 + 256 * u);
+    b[u] = __builtin_nontemporal_load(k2 + 256 * u);
   }
+  double m = 0.0;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int64_t e = e0 + 256 * u;
-    if (e < n2) __builtin_nontemporal_store(a[u] - 0.5 * b[u], j2 + e);
+  for (int u = 0; u < 8; ++u) {
+    const dvec2 v = a[u] - 0.5 * b[u];
+    __builtin_nontemporal_store(v, j2 + 256 * u);
+    m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
   }
+  if (tmax) tile_block_max(m, tmax + t);
 }
 
 // Work item m of the SYMV: tile row R, segment s covering tiles
@@ -912,24 +928,30 @@ using namespace qcmd;
 
 extern "C" {
 
-int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* stream) {
+int qc_sm_assemble_maxabs(int kind, const double* Jsm, double* out, int64_t nbf, double* tmax,
+                          void* stream) {
   if (kind != 0 && kind != 1) return set_error("qc_sm_assemble: kind %d", kind);
   if (nbf <= 0) return set_error("qc_sm_assemble: nbf %lld", (long long)nbf);
+  if (!Jsm || !out) return set_error("qc_sm_assemble: null supermatrix");
   if (Jsm == out) return set_error("qc_sm_assemble: in-place assembly is not supported");
   const int64_t T = sm_tiles_per_dim(nbf);
   const int64_t ntiles = T * (T + 1) / 2;
   hipLaunchKernelGGL(sm_assemble_kernel, grid_1d(ntiles, 256), dim3(256), 0, (hipStream_t)stream,
-                     kind, Jsm, out, nbf, ntiles);
+                     kind, Jsm, out, nbf, ntiles, tmax);
   return check(hipGetLastError(), "qc_sm_assemble launch");
 }
 
-int qc_sm_p_from_jk(double* J, const double* K, int64_t nbf, void* stream) {
+int qc_sm_assemble(int kind, const double* Jsm, double* out, int64_t nbf, void* stream) {
+  return qc_sm_assemble_maxabs(kind, Jsm, out, nbf, nullptr, stream);
+}
+
+int qc_sm_p_from_jk(double* J, const double* K, int64_t nbf, double* tmax, void* stream) {
   if (nbf <= 0) return set_error("qc_sm_p_from_jk: nbf %lld", (long long)nbf);
   if (!J || !K || J == K) return set_error("qc_sm_p_from_jk: J and K must be distinct buffers");
   const int64_t T = sm_tiles_per_dim(nbf);
-  const int64_t n2 = ((T * (T + 1) / 2) << (2 * SM_LOGB)) / 2;
-  hipLaunchKernelGGL(sm_p_from_jk_kernel, grid_1d((n2 + 1023) / 1024, 256), dim3(256), 0,
-                     (hipStream_t)stream, J, K, n2);
+  const int64_t ntiles = T * (T + 1) / 2;
+  hipLaunchKernelGGL(sm_p_from_jk_kernel, grid_1d(ntiles, 256), dim3(256), 0, (hipStream_t)stream,
+                     J, K, ntiles, tmax);
   return check(hipGetLastError(), "qc_sm_p_from_jk launch");
 }
 

@@ -58,7 +58,8 @@ _SIGS = {
     "qc_one_electron": [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                         c_int, c_void_p, c_void_p, c_i64, c_void_p, c_void_p],
     "qc_sm_assemble": [c_int, c_void_p, c_void_p, c_i64, c_void_p],
-    "qc_sm_p_from_jk": [c_void_p, c_void_p, c_i64, c_void_p],
+    "qc_sm_assemble_maxabs": [c_int, c_void_p, c_void_p, c_i64, c_void_p, c_void_p],
+    "qc_sm_p_from_jk": [c_void_p, c_void_p, c_i64, c_void_p, c_void_p],
     "qc_symv": [c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "qc_symv_tiles": [c_void_p, c_i64, c_int, c_void_p, c_void_p, c_void_p],
     "qc_symv_reduce": [c_i64, c_int, c_void_p, c_void_p, c_void_p],

@@ -100,6 +100,10 @@ class ERIStore:
         torch.cuda.synchronize(self.dev)
         self.eri_seconds = time.perf_counter() - t0
         self.P = self.K = None
+        # per-tile max |element| of P and K, written by the kernels that
+        # assemble them (what compact() needs, without another read of the
+        # matrix); dropped once used
+        self._tmax = {}
         self._work, self._x, self._y, self._xm = {}, {}, {}, {}
         self.inc = None
 
@@ -113,9 +117,9 @@ class ERIStore:
         if self.P is None and self.K is None:
             self.ensure_P()
         if isinstance(self.P, torch.Tensor):
-            self.P = self.compact(self.P, **kw)
+            self.P = self.compact(self.P, tmax=self._tmax.pop("P", None), **kw)
         if isinstance(self.K, torch.Tensor):
-            self.K = self.compact(self.K, **kw)
+            self.K = self.compact(self.K, tmax=self._tmax.pop("K", None), **kw)
         if isinstance(self.J, torch.Tensor) and self.K is not None:
             self.J = self.compact(self.J, **kw)
         return self
@@ -131,16 +135,21 @@ class ERIStore:
         if self.P is None:
             self._assemble_from_J("P")
             self.P = torch.empty_like(self.J)
-            nat.call("qc_sm_assemble", 0, nat.ptr(self.J), nat.ptr(self.P), self.nbf,
-                     nat.stream_ptr())
+            self._tmax["P"] = self._tile_vector()
+            nat.call("qc_sm_assemble_maxabs", 0, nat.ptr(self.J), nat.ptr(self.P), self.nbf,
+                     nat.ptr(self._tmax["P"]), nat.stream_ptr())
         return self.P
+
+    def _tile_vector(self):
+        return torch.empty(self.T * (self.T + 1) // 2, dtype=torch.float64, device=self.dev)
 
     def ensure_K(self):
         if self.K is None:
             self._assemble_from_J("K")
             self.K = torch.empty_like(self.J)
-            nat.call("qc_sm_assemble", 1, nat.ptr(self.J), nat.ptr(self.K), self.nbf,
-                     nat.stream_ptr())
+            self._tmax["K"] = self._tile_vector()
+            nat.call("qc_sm_assemble_maxabs", 1, nat.ptr(self.J), nat.ptr(self.K), self.nbf,
+                     nat.ptr(self._tmax["K"]), nat.stream_ptr())
         return self.K
 
     def assemble_p_k(self):
@@ -152,7 +161,9 @@ class ERIStore:
         K = self.ensure_K()
         if self.P is None:
             self._assemble_from_J("P")
-            nat.call("qc_sm_p_from_jk", nat.ptr(self.J), nat.ptr(K), self.nbf, nat.stream_ptr())
+            self._tmax["P"] = self._tile_vector()
+            nat.call("qc_sm_p_from_jk", nat.ptr(self.J), nat.ptr(K), self.nbf,
+                     nat.ptr(self._tmax["P"]), nat.stream_ptr())
             self.P, self.J = self.J, None
         return self.P, K
 
@@ -162,16 +173,21 @@ class ERIStore:
         self.J = None
 
     def compact(self, M: torch.Tensor, tau_f32: float = 1e-4, tau_zero: float = 1e-14,
-                tau_i16: float = 0.0, tau_i8: float = 0.0) -> CompactSupermatrix:
+                tau_i16: float = 0.0, tau_i8: float = 0.0,
+                tmax: torch.Tensor | None = None) -> CompactSupermatrix:
         """Mixed-precision, zero-skipping copy of supermatrix M (formats by
         tile max |element|: see CompactSupermatrix; tau_i16 = tau_i8 = 0 keeps
-        the integer formats off)."""
+        the integer formats off).  tmax: M's per-tile max |element| if the
+        caller has it (from the kernel that assembled M), else computed."""
         if tau_i8 > max(tau_i16, tau_f32) or tau_i16 > tau_f32:
             raise ValueError("tile thresholds must satisfy tau_i8 <= tau_i16 <= tau_f32")
         T = self.T
         nt = T * (T + 1) // 2
-        tmax = torch.empty(nt, dtype=torch.float64, device=self.dev)
-        nat.call("qc_sm_tile_maxabs", nat.ptr(M), self.nbf, nat.ptr(tmax), nat.stream_ptr())
+        if tmax is None:
+            tmax = torch.empty(nt, dtype=torch.float64, device=self.dev)
+            nat.call("qc_sm_tile_maxabs", nat.ptr(M), self.nbf, nat.ptr(tmax), nat.stream_ptr())
+        elif tmax.shape != (nt,) or tmax.dtype != torch.float64:
+            raise ValueError(f"tmax must be float64 of shape ({nt},)")
         # formats, sizes and pool offsets on the device; only the five format
         # counts come back to the host (to size the pool)
         fd = tile_formats(tmax, tau_f32, tau_zero, tau_i16, tau_i8)

@@ -172,6 +172,38 @@ def test_store_uhf_fock_from_p_and_k_matches_oracle(case):
     np.testing.assert_allclose(st.coulomb(Dat).cpu().numpy(), scf.coulomb(Da, ref["ERI"]), atol=1e-11)
 
 
+def test_assembly_tile_maxima_equal_tile_maxabs_of_the_result(case):
+    """The per-tile max |element| the assembly kernels write beside P (kind 0),
+    K (kind 1) and P = J - K/2 in place equals qc_sm_tile_maxabs over the
+    assembled matrix bit for bit, so a compaction from them is the same
+    compaction (directory and pool) as one that reads the matrix again."""
+    from jax_qc import _native as nat
+    from jax_qc.integrals.store import ERIStore
+
+    mol, bs, ref = case
+    st = ERIStore(bs, tau=0.0)
+
+    def maxabs(M):
+        t = st._tile_vector()
+        nat.call("qc_sm_tile_maxabs", nat.ptr(M), st.nbf, nat.ptr(t), nat.stream_ptr())
+        return t
+
+    st.ensure_P()
+    st.ensure_K()
+    assert torch.equal(st._tmax["P"], maxabs(st.P))
+    assert torch.equal(st._tmax["K"], maxabs(st.K))
+    assert float(st._tmax["K"].max()) > 0.0
+    st.P = None
+    st.assemble_p_k()
+    assert torch.equal(st._tmax["P"], maxabs(st.P))
+    kw = dict(tau_f32=1e-2, tau_i16=3e-5, tau_i8=1e-7)
+    for name, M in (("P", st.P), ("K", st.K)):
+        a = st.compact(M, tmax=st._tmax[name], **kw)
+        b = st.compact(M, **kw)
+        assert torch.equal(a.fmt, b.fmt) and torch.equal(a.off, b.off), name
+        assert torch.equal(a.pool, b.pool) and torch.equal(a.tmax, b.tmax), name
+
+
 def test_dense_jk_kernel(case):
     from jax_qc.scf.fock import _dense_jk
     from oracle import scf
