@@ -7,7 +7,7 @@ functions), energy + analytic gradient through two paths:
 * FP64 ("plain", the drop-in): Schwarz screening 1e-12 and nothing else;
   FP64 ERIs, FP64 supermatrices, full Fock builds, FP64 gradient at 1e-12.
 * mixed: ERI primitive sums in FP32 for quartets with Schwarz bound < 1e-4,
-  supermatrix tiles below max|element| 1e-7 / 3e-5 / 1e-2 stored as int8 /
+  supermatrix tiles below max|element| 3e-7 / 1e-4 / 1e-2 stored as int8 /
   int16 (per-tile scale) / FP32, primitive pairs
   with a Cauchy-Schwarz bound below 1e-12 left out (1e-10 for the derivative
   integrals), incremental Fock builds
@@ -66,11 +66,11 @@ MEASURED_TRAFFIC = {"fp64": (1639096.8 * 2.0 + 16188.0) * 1024,
                     "mixed": (1150726.7 * 2.0 + 16188.0) * 1024}
 # the same for the P tile pass (symv_group_kernel<true,true> over the
 # compacted P = J - K/2 pool) of the (H2O)_16 mixed workload (int8 / int16 /
-# FP32 / FP64 tiles): FETCH 11,491,464 KiB x 2.00 + WRITE 255,154 KiB per
+# FP32 / FP64 tiles): FETCH 10,772,866 KiB x 2.00 + WRITE 255,154 KiB per
 # launch (mean of 5 launches, profiles/fock_and_energy_gradient_water16_uhf.md),
-# 1.034 x the 23.0 GB algorithmic pool bytes (the rest: row and column
+# 1.035 x the 21.6 GB algorithmic pool bytes (the rest: row and column
 # partials, x and the tile directory)
-TRAFFIC_C5_P = (11491464 * 2 + 255154) * 1024
+TRAFFIC_C5_P = (10772866 * 2 + 255154) * 1024
 
 
 def _cpu_info():
