@@ -825,8 +825,9 @@ def main():
                          "both paths' gradients (JSON lines; no benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the quartet "
-                                   "list is design-only (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; sharding the quartet list and "
+                                   "the supermatrix tile rows over several GPUs is not implemented",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "batch":
         return run_batch(args)
