@@ -20,6 +20,7 @@ This is the integral-storage counterpart of the reference's dense
 from __future__ import annotations
 
 import math
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -35,6 +36,47 @@ TILE_ZERO, TILE_F32, TILE_F64, TILE_I16, TILE_I8 = 0, 1, 2, 3, 4
 # a 16-byte header (the tile's FP64 scale, then padding)
 TILE_BYTES = {TILE_ZERO: 0, TILE_F32: 4 * 4096, TILE_F64: 8 * 4096,
               TILE_I16: 16 + 2 * 4096, TILE_I8: 16 + 4096}
+
+
+# FP64 supermatrix buffers kept for the next store of the same size (J and
+# the K or P assembled beside it).  Back-to-back evaluations at one basis size
+# (a geometry optimisation) then allocate no new 10-100 GB blocks: torch's
+# allocator leaves small long-lived tensors inside a freed supermatrix's
+# segment, so the next store's J needed a fresh device allocation, which
+# took 0.4-2 s per (H2O)_16 evaluation.  A buffer is free when nothing but
+# this list refers to it; release_supermatrix_buffers() hands the memory back.
+_SM_BUFFERS: list = []
+SM_BUFFERS_MAX = 2
+
+
+def _sm_buffer(numel: int, dev: torch.device, zero: bool) -> torch.Tensor:
+    """A float64 vector of numel elements on dev, from the kept buffers when
+    one of that size is free (zeroed if asked), else newly allocated (and
+    kept while fewer than SM_BUFFERS_MAX are)."""
+    for i in range(len(_SM_BUFFERS)):
+        # a free buffer has two references: the list and getrefcount's argument
+        if (sys.getrefcount(_SM_BUFFERS[i]) == 2 and _SM_BUFFERS[i].numel() == numel
+                and _SM_BUFFERS[i].device == dev):
+            t = _SM_BUFFERS[i]
+            return t.zero_() if zero else t
+    for i in reversed(range(len(_SM_BUFFERS))):  # free buffers of other sizes go
+        if sys.getrefcount(_SM_BUFFERS[i]) == 2 and _SM_BUFFERS[i].numel() != numel:
+            del _SM_BUFFERS[i]
+    if zero:
+        t = torch.zeros(numel, dtype=torch.float64, device=dev)
+    else:
+        t = torch.empty(numel, dtype=torch.float64, device=dev)
+    if len(_SM_BUFFERS) < SM_BUFFERS_MAX:
+        _SM_BUFFERS.append(t)
+    return t
+
+
+def release_supermatrix_buffers() -> None:
+    """Drop the kept FP64 supermatrix buffers (those no store still uses
+    are freed) and return torch's cached device memory."""
+    _SM_BUFFERS.clear()
+    if torch.cuda.is_available():
+        torch.cuda.empty_cache()
 
 
 @dataclass
@@ -95,7 +137,7 @@ class ERIStore:
         self.T = eng.sm_tiles_per_dim(self.nbf)
         self.npad = self.T * eng.SM_B
         t0 = time.perf_counter()
-        self.J = torch.zeros(eng.sm_numel(self.nbf), dtype=torch.float64, device=self.dev)
+        self.J = _sm_buffer(eng.sm_numel(self.nbf), self.dev, zero=True)
         eng.run_eri(self.pt, self.plan, eng.MODE_SUPER, self.J)
         torch.cuda.synchronize(self.dev)
         self.eri_seconds = time.perf_counter() - t0
@@ -134,7 +176,7 @@ class ERIStore:
     def ensure_P(self):
         if self.P is None:
             self._assemble_from_J("P")
-            self.P = torch.empty_like(self.J)
+            self.P = _sm_buffer(self.J.numel(), self.dev, zero=False)
             self._tmax["P"] = self._tile_vector()
             nat.call("qc_sm_assemble_maxabs", 0, nat.ptr(self.J), nat.ptr(self.P), self.nbf,
                      nat.ptr(self._tmax["P"]), nat.stream_ptr())
@@ -146,7 +188,7 @@ class ERIStore:
     def ensure_K(self):
         if self.K is None:
             self._assemble_from_J("K")
-            self.K = torch.empty_like(self.J)
+            self.K = _sm_buffer(self.J.numel(), self.dev, zero=False)
             self._tmax["K"] = self._tile_vector()
             nat.call("qc_sm_assemble_maxabs", 1, nat.ptr(self.J), nat.ptr(self.K), self.nbf,
                      nat.ptr(self._tmax["K"]), nat.stream_ptr())
