@@ -44,7 +44,9 @@ TILE_BYTES = {TILE_ZERO: 0, TILE_F32: 4 * 4096, TILE_F64: 8 * 4096,
 # allocator leaves small long-lived tensors inside a freed supermatrix's
 # segment, so the next store's J needed a fresh device allocation, which
 # took 0.4-2 s per (H2O)_16 evaluation.  A buffer is free when nothing but
-# this list refers to it; release_supermatrix_buffers() hands the memory back.
+# this list refers to it (views do not count: a caller that keeps a view of a
+# store's FP64 supermatrix past the store's life must copy it instead);
+# release_supermatrix_buffers() hands the memory back.
 _SM_BUFFERS: list = []
 SM_BUFFERS_MAX = 2
 
